@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the particle-life physics step (Physics.update()) on one MI355X.
+
+Workload: the single large 2D world -- N = 1,000,000 particles, 8 types, rmax 0.01
+in [-1,1)^2 (0.005 in the reference's unit box, ~78.5 interacting neighbours),
+beta 0.3, friction half-life 0.04 s, dt 0.01 s, periodic.  One step = one
+Physics.update(): cell keys, stable sort, gather, fused force + integrate.
+
+Prints ONE JSON line.  `value` is pair-interactions/s (ordered interacting
+pairs per step, counted exactly by the kernel, times steps/s) with the state
+resident in HBM; the timed region is K steps between two HIP events on the
+library's stream, bracketed by synchronisations.
+
+roofline: the force+integrate kernel (the dominant one) against the FP32
+vector peak, with algorithmic work of 22 FLOP per interacting 2D pair.
+cpu_baseline: the FP64 C restatement (oracle/) on this host's cores, a bounded
+sample of the same world.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "particle-life-app_amd"))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR_2D = 22          # algorithmic FLOPs per interacting pair (2D)
+BYTES_PER_PARTICLE_2D = 36     # read x,y,vx,vy,type + write x,y,vx,vy
+PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--profile-steps", type=int, default=20)
+    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, available cores)")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_threads(requested: int) -> int:
+    if requested > 0:
+        return requested
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(world, steps: int, threads: int) -> dict:
+    """FP64 restatement of Physics.update() on the host, same world."""
+    import oracle
+
+    S = oracle.Settings(matrix=world.matrix, rmax=world.rmax, friction=world.friction,
+                        dt=world.dt, beta=world.beta, force=world.force, wrap=world.wrap,
+                        dim=world.dim)
+    pos, vel, typ = world.pos, world.vel, world.types
+    ids = None
+    pairs = 0
+    t_total = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r = oracle.step(S, pos, vel, typ, ids=ids, nthreads=threads, debug=True)
+        t_total += time.perf_counter() - t0
+        pairs += r["pair_count"]
+        pos, vel, typ, ids = r["pos"], r["vel"], r["types"], r["ids"]
+    return {
+        "value": pairs / t_total,
+        "unit": "pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the same N={world.n} world from its initial state, "
+                  f"FP64 C restatement, {threads} threads ({t_total:.2f} s)",
+        "steps_per_s": steps / t_total,
+    }
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the world is stepped "
+              "on a single MI355X and multi-GPU sharding is not part of this build.",
+              file=sys.stderr)
+        return 2
+
+    from particle_life import Engine, worlds
+
+    w = worlds.config2(args.n)
+    eng = Engine(0)
+    eng.load_world(w)
+    eng.step(args.warmup)
+    eng.sync()
+
+    # timed region: K steps between two events on the library's stream
+    eng.sync()
+    t0 = time.perf_counter()
+    ms = eng.time_steps(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    pairs_last = eng.pair_count()
+
+    # per-phase timing (events around each phase) on further steps
+    prof = eng.profile(args.profile_steps)
+    pairs = prof["pairs"]
+    steps_per_s = args.steps / (ms / 1e3)
+    value = pairs_last * steps_per_s
+
+    force_s = prof["force_ms"] / 1e3
+    achieved_tflops = FLOP_PER_PAIR_2D * pairs / force_s / 1e12
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "force_c2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "pair-interactions/s & steps/s at N=1M 2D",
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "steps_per_s": steps_per_s,
+        "particle_steps_per_s": w.n * steps_per_s,
+        "pairs_per_step": pairs_last,
+        "wall_s": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, f64 force law, u32 fixed-point positions",
+        "data": "synthetic (seeded SplitMix64 world)",
+        "config": {
+            "workload": "single large 2D world: N=1,000,000, 8 types, rmax 0.01 in [-1,1)^2 "
+                        "(0.005 unit box), beta 0.3, friction half-life 0.04 s, dt 0.01 s, wrap",
+            "n": w.n, "ntypes": w.ntypes, "rmax_unit_box": w.rmax, "dt": w.dt,
+        },
+        "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
+        "roofline": {
+            "kernel": "k_force<2,true,false>",
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "algorithmic": f"{FLOP_PER_PAIR_2D} FLOP/pair x {pairs} pairs per launch",
+            "hbm_frac_algorithmic": BYTES_PER_PARTICLE_2D * w.n * steps_per_s / 1e9 / PEAK_HBM_GBS,
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(w, args.cpu_steps, cpu_threads(args.cpu_threads))
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    eng.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

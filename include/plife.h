@@ -1,0 +1,135 @@
+/*
+ * plife.h -- C ABI of the MI355X (gfx950) particle-life physics library.
+ *
+ * This is the drop-in boundary for the reference's per-frame physics step,
+ * Physics.update() (src/main/java/com/particle_life/backend/Physics.java:112-134).
+ * A Java GpuPhysics (JNI) or any other host binds these entry points; plain
+ * pointers and sizes only.  Host arrays are caller-owned and only read/written
+ * during the call; device buffers are owned by the handle.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: PLIFE_OK (0) on success, a
+ *     negative PLIFE_ERR_* code on failure (plife_last_error() then returns a
+ *     thread-local message).  Nothing throws across the ABI.
+ *   - A handle is not thread-safe: all calls come from one thread (the app's
+ *     physics Loop thread), except plife_force_update_stop().
+ *   - Coordinates are in the reference's unit box: [0,1) periodic (wrap) or
+ *     [0,1] clamped.  Positions and velocities cross the boundary as FP64
+ *     (x,y,z) triples, the layout of app/PhysicsSnapshot.java:11-13; the
+ *     device holds positions as 32-bit fixed point (x = k * 2^-32) and
+ *     velocities as FP32.
+ */
+#ifndef PLIFE_H
+#define PLIFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLIFE_OK 0
+#define PLIFE_STOPPED 1          /* plife_update returned early after a stop request */
+#define PLIFE_ERR_ARGUMENT (-1)  /* Java: IllegalArgumentException */
+#define PLIFE_ERR_STATE (-2)     /* call order (e.g. update before settings) */
+#define PLIFE_ERR_DEVICE (-3)    /* HIP runtime failure; Java: RuntimeException */
+#define PLIFE_ERR_MEMORY (-4)    /* device allocation failed */
+
+typedef struct plife_physics plife_physics;
+
+/* PhysicsSettings (backend/PhysicsSettings.java:8-38) plus the accelerator's
+ * beta (app/Main.java:276) and the world dimension.  dim 2 is the reference;
+ * dim 3 is an extension (27-cell stencil). */
+typedef struct {
+    int32_t dim;       /* 2 or 3 */
+    int32_t wrap;      /* 1 periodic [0,1), 0 clamped [0,1] */
+    double rmax;       /* interaction radius, 0 < rmax <= 1; cell size = rmax */
+    double friction;   /* per-1/60 s coefficient; factor = friction^(60*dt) */
+    double force;      /* force scale */
+    double dt;         /* time step (s) */
+    double beta;       /* repulsion zone, 0 < beta < 1 */
+} plife_settings;
+
+/* Per-phase device timings from plife_profile (milliseconds per step). */
+typedef struct {
+    double step_ms;     /* wall time per step, event-to-event over all steps */
+    double bin_ms;      /* cell keys + cell histogram + cell scan */
+    double sort_ms;     /* stable radix sort passes */
+    double gather_ms;   /* SoA reorder into cell order */
+    double force_ms;    /* force + integrate kernel (average launch duration) */
+    int64_t pairs;      /* interacting ordered pairs in the last step */
+    int32_t steps;
+} plife_timing;
+
+/* Replaces the device-side half of `new Physics(...)` (Physics.java:65-80)
+ * and Physics.kill() (Physics.java:156-158). */
+int plife_create(int32_t device, plife_physics **out);
+int plife_destroy(plife_physics *h);
+
+/* Reads PhysicsSettings + Matrix (backend/Matrix.java:3-8, DefaultMatrix.java:39-41,
+ * row = type of the acted-on particle) + beta.  Cheap to call every frame
+ * (app/Main.java:292-295 overwrites dt per frame); the matrix is re-uploaded
+ * only when it changed.  ntypes in [1, 64]. */
+int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
+                       const double *matrix);
+
+/* Uploads Particle[] (backend/Particle.java:5-9): pos/vel are [3n] (x,y,z),
+ * types [n] in [0, ntypes).  Positions are first brought into range as
+ * Physics.ensurePosition does (Physics.java:499-505).  Particle ids are the
+ * array indices of this call and travel with the particles. */
+int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const double *vel,
+                        const int32_t *types);
+
+/* Downloads the state in the current device array order (cell-sorted after
+ * a step, like Physics.particles after makeContainers' swap).  Any output
+ * pointer may be NULL.  Synchronises the handle's stream. */
+int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
+                        int32_t *ids);
+
+int plife_get_count(plife_physics *h, int32_t *n);
+
+/* Physics.update() (Physics.java:112-134) applied nsteps times.  Enqueues the
+ * steps on the handle's stream and returns; plife_synchronize or a download
+ * waits.  Returns PLIFE_STOPPED if plife_force_update_stop() arrived between
+ * steps (the state is then at a step boundary). */
+int plife_update(plife_physics *h, int32_t nsteps);
+
+/* Physics.forceUpdateStop() (Physics.java:149-151).  The only call allowed
+ * from another thread; honoured between steps. */
+int plife_force_update_stop(plife_physics *h);
+
+int plife_synchronize(plife_physics *h);
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char *plife_last_error(void);
+
+/* Parity exports of the last step (Physics.java:309-354): cell key of each
+ * particle in the pre-sort order, the reference's containers array (end
+ * offsets, ncells = nx^dim), and the permutation (sorted slot -> pre-sort
+ * index).  Any pointer may be NULL.  ncells must equal the grid's cell count. */
+int plife_get_cells(plife_physics *h, int32_t *keys, int32_t *cell_end, int32_t *perm,
+                    int32_t ncells);
+
+/* Cells per axis of the current settings: (int)floor(1/rmax). */
+int plife_get_grid(plife_physics *h, int32_t *nx, int32_t *ncells);
+
+/* Interacting ordered pairs (0 < d^2 <= rmax^2) counted in the last step. */
+int plife_get_pair_count(plife_physics *h, int64_t *pairs);
+
+/* Debug mode: when on, each step also records per particle (sorted slot) the
+ * number of interacting neighbours and a 64-bit sum of a mix of their ids;
+ * plife_get_neighbor_summary downloads them.  Off by default. */
+int plife_set_debug(plife_physics *h, int32_t on);
+int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash);
+
+/* Runs nsteps steps with HIP events around each phase and fills *out. */
+int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out);
+
+/* Runs nsteps steps and returns the device time between two events on the
+ * handle's stream bracketing them (ms). */
+int plife_time_steps(plife_physics *h, int32_t nsteps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,787 @@
+// plife.hip -- C ABI (include/plife.h) of the gfx950 particle-life physics step.
+//
+// Host orchestration of one Physics.update() (backend/Physics.java:112-134):
+//   bin (keys + cell histogram + scan) -> stable radix sort -> gather -> force+integrate,
+// all enqueued on the handle's own stream.  No torch, no CPU fallback: every
+// step runs the HIP kernels of plife_kernels.hpp and plife_force.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plife.h"
+#include "plife_force.hpp"
+
+using namespace plife;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(PLIFE_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                  \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess)                                                           \
+            return fail(PLIFE_ERR_DEVICE, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kMaxTypes = 256;        // the app caps the matrix size at 256 (app/Main.java:745)
+constexpr int kMaxTypesLds = 32;      // matrix staged in LDS up to 32x32 (8 KB of FP64)
+constexpr int kListCap = 64;          // per-thread candidate list entries (u16) in LDS
+constexpr long long kMaxCells = 1LL << 28;
+constexpr int kMaxLds = 65536;
+
+template <typename T>
+int dev_alloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PLIFE_ERR_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    return PLIFE_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+struct plife_physics {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // settings
+    bool have_settings = false;
+    plife_settings st{};
+    int ntypes = 0;
+    std::vector<double> matrix_host;
+    double *d_matrix = nullptr;
+
+    // particle state: positions ping-pong P[2], the rest A[2]
+    int n = 0, cap = 0;
+    uint32_t *px[2] = {}, *py[2] = {}, *pz[2] = {};
+    float *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
+    int32_t *type[2] = {}, *id[2] = {};
+    int pc = 0, ac = 0;
+
+    // binning / sort scratch
+    uint32_t *keys0 = nullptr, *kA = nullptr, *kB = nullptr, *vA = nullptr, *vB = nullptr;
+    uint32_t *perm = nullptr;          // points at vA or vB after a step
+    long long cells_cap = 0;
+    uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_end = nullptr;
+    size_t hist_cap = 0, bsum_cap = 0;
+    uint32_t *hist = nullptr, *bsum = nullptr;
+    unsigned long long *d_pairs = nullptr;
+
+    // grid of the last step
+    bool stepped = false;
+    int last_nx = 0;
+    long long last_ncells = 0;
+
+    // debug
+    int debug = 0;
+    int dbg_cap = 0;
+    int32_t *dbg_count = nullptr;
+    unsigned long long *dbg_hash = nullptr;
+
+    std::atomic<int> stop{0};
+
+    PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
+    AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
+};
+
+namespace {
+
+int set_device(plife_physics *h)
+{
+    HIP_TRY(hipSetDevice(h->device));
+    return PLIFE_OK;
+}
+
+void free_state(plife_physics *h)
+{
+    for (int k = 0; k < 2; ++k) {
+        dev_free(h->px[k]);
+        dev_free(h->py[k]);
+        dev_free(h->pz[k]);
+        dev_free(h->vx[k]);
+        dev_free(h->vy[k]);
+        dev_free(h->vz[k]);
+        dev_free(h->type[k]);
+        dev_free(h->id[k]);
+    }
+    dev_free(h->keys0);
+    dev_free(h->kA);
+    dev_free(h->kB);
+    dev_free(h->vA);
+    dev_free(h->vB);
+    h->perm = nullptr;
+    h->cap = 0;
+}
+
+int ensure_capacity(plife_physics *h, int n)
+{
+    if (n <= h->cap && h->cap > 0) return PLIFE_OK;
+    free_state(h);
+    const int cap = std::max(n, 1);
+    int rc;
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = dev_alloc(&h->px[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->py[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->pz[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->vx[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->vy[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->vz[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->type[k], cap))) return rc;
+        if ((rc = dev_alloc(&h->id[k], cap))) return rc;
+    }
+    if ((rc = dev_alloc(&h->keys0, cap))) return rc;
+    if ((rc = dev_alloc(&h->kA, cap))) return rc;
+    if ((rc = dev_alloc(&h->kB, cap))) return rc;
+    if ((rc = dev_alloc(&h->vA, cap))) return rc;
+    if ((rc = dev_alloc(&h->vB, cap))) return rc;
+    h->cap = cap;
+    return PLIFE_OK;
+}
+
+int ensure_cells(plife_physics *h, long long ncells)
+{
+    if (ncells <= h->cells_cap) return PLIFE_OK;
+    dev_free(h->cell_count);
+    dev_free(h->cell_start);
+    dev_free(h->cell_end);
+    h->cells_cap = 0;
+    int rc;
+    if ((rc = dev_alloc(&h->cell_count, (size_t)ncells))) return rc;
+    if ((rc = dev_alloc(&h->cell_start, (size_t)ncells))) return rc;
+    if ((rc = dev_alloc(&h->cell_end, (size_t)ncells))) return rc;
+    h->cells_cap = ncells;
+    return PLIFE_OK;
+}
+
+int ensure_u32(uint32_t **p, size_t *capv, size_t need)
+{
+    if (need <= *capv && *p) return PLIFE_OK;
+    dev_free(*p);
+    *capv = 0;
+    int rc = dev_alloc(p, need);
+    if (rc) return rc;
+    *capv = need;
+    return PLIFE_OK;
+}
+
+int ensure_debug(plife_physics *h)
+{
+    if (!h->debug || h->dbg_cap >= std::max(h->n, 1)) return PLIFE_OK;
+    dev_free(h->dbg_count);
+    dev_free(h->dbg_hash);
+    h->dbg_cap = 0;
+    int rc;
+    const int cap = std::max(h->n, 1);
+    if ((rc = dev_alloc(&h->dbg_count, cap))) return rc;
+    if ((rc = dev_alloc(&h->dbg_hash, cap))) return rc;
+    h->dbg_cap = cap;
+    return PLIFE_OK;
+}
+
+// device-wide scan: excl[i] = sum in[0..i), incl[i] = sum in[0..i]; in may alias excl.
+int scan_u32(plife_physics *h, const uint32_t *in, int n, uint32_t *excl, uint32_t *incl)
+{
+    if (n <= 0) return PLIFE_OK;
+    const int nb = ceil_div(n, kScanTile);
+    int rc = ensure_u32(&h->bsum, &h->bsum_cap, (size_t)nb);
+    if (rc) return rc;
+    k_scan_reduce<<<nb, kBlock, 0, h->stream>>>(in, n, h->bsum);
+    k_scan_single<<<1, kBlock, 0, h->stream>>>(h->bsum, nb);
+    k_scan_down<<<nb, kBlock, 0, h->stream>>>(in, n, h->bsum, excl, incl);
+    LAUNCH_CHECK();
+    return PLIFE_OK;
+}
+
+template <int D>
+int radix_pass(plife_physics *h, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+               uint32_t *vout, int n, int shift, int ntiles)
+{
+    k_radix_hist<D><<<ntiles, kBlock, 0, h->stream>>>(kin, n, shift, ntiles, h->hist);
+    int rc = scan_u32(h, h->hist, (1 << D) * ntiles, h->hist, nullptr);
+    if (rc) return rc;
+    k_radix_scatter<D><<<ntiles, kBlock, 0, h->stream>>>(kin, vin, kout, vout, h->hist, n, shift,
+                                                         ntiles);
+    LAUNCH_CHECK();
+    return PLIFE_OK;
+}
+
+// Stable sort of (keys0[i], i) by key; leaves the permutation in h->perm.
+int radix_sort(plife_physics *h, int n, long long ncells)
+{
+    int bits = 0;
+    while ((1LL << bits) < ncells) ++bits;
+    if (bits == 0) bits = 1;
+    const int passes = (bits + 7) / 8;
+    const int D = (bits + passes - 1) / passes;
+    const int ntiles = ceil_div(n, kRadixTile);
+    int rc = ensure_u32(&h->hist, &h->hist_cap, (size_t)(1 << D) * (size_t)ntiles);
+    if (rc) return rc;
+    const uint32_t *kin = h->keys0;
+    const uint32_t *vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const bool last = p == passes - 1;
+        uint32_t *kout = last ? nullptr : ((p & 1) ? h->kB : h->kA);
+        uint32_t *vout = (p & 1) ? h->vB : h->vA;
+        const int shift = p * D;
+        switch (D) {
+            case 1: rc = radix_pass<1>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 2: rc = radix_pass<2>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 3: rc = radix_pass<3>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 4: rc = radix_pass<4>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 5: rc = radix_pass<5>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 6: rc = radix_pass<6>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            case 7: rc = radix_pass<7>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+            default: rc = radix_pass<8>(h, kin, vin, kout, vout, n, shift, ntiles); break;
+        }
+        if (rc) return rc;
+        kin = kout;
+        vin = vout;
+    }
+    LAUNCH_CHECK();
+    h->perm = const_cast<uint32_t *>(vin);
+    return PLIFE_OK;
+}
+
+struct Grid {
+    int nx;
+    long long ncells;
+};
+
+Grid grid_of(const plife_settings &s)
+{
+    Grid g;
+    g.nx = (int)std::floor(1.0 / s.rmax);     // Physics.java:82-85, 312-313
+    g.ncells = s.dim == 3 ? (long long)g.nx * g.nx * g.nx : (long long)g.nx * g.nx;
+    return g;
+}
+
+template <int DIM, bool WRAP, bool DEBUG>
+void launch_force(plife_physics *h, const ForceArgs &a, int grid, size_t lds)
+{
+    k_force<DIM, WRAP, DEBUG><<<grid, kBlock, lds, h->stream>>>(a);
+}
+
+int launch_force_any(plife_physics *h, const ForceArgs &a, int grid, size_t lds)
+{
+    const int dim = h->st.dim;
+    const bool wrap = h->st.wrap != 0, dbg = h->debug != 0;
+    if (dim == 2) {
+        if (wrap) dbg ? launch_force<2, true, true>(h, a, grid, lds) : launch_force<2, true, false>(h, a, grid, lds);
+        else dbg ? launch_force<2, false, true>(h, a, grid, lds) : launch_force<2, false, false>(h, a, grid, lds);
+    } else {
+        if (wrap) dbg ? launch_force<3, true, true>(h, a, grid, lds) : launch_force<3, true, false>(h, a, grid, lds);
+        else dbg ? launch_force<3, false, true>(h, a, grid, lds) : launch_force<3, false, false>(h, a, grid, lds);
+    }
+    LAUNCH_CHECK();
+    return PLIFE_OK;
+}
+
+// Events bracketing the phases of one step (optional).
+struct PhaseEvents {
+    hipEvent_t e[5];
+};
+
+int record(plife_physics *h, PhaseEvents *ev, int k)
+{
+    if (ev) HIP_TRY(hipEventRecord(ev->e[k], h->stream));
+    return PLIFE_OK;
+}
+
+int do_step(plife_physics *h, PhaseEvents *ev)
+{
+    const plife_settings &s = h->st;
+    const Grid gr = grid_of(s);
+    const int n = h->n;
+    int rc;
+    if ((rc = ensure_cells(h, gr.ncells))) return rc;
+    if ((rc = ensure_debug(h))) return rc;
+    const int nc = (int)gr.ncells;
+
+    if ((rc = record(h, ev, 0))) return rc;
+    // 1. bin: cell keys + histogram, scan into start/end offsets (Physics.java:319-340)
+    k_fill_u32<<<ceil_div(nc, kBlock), kBlock, 0, h->stream>>>(h->cell_count, 0u, nc);
+    k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 1);
+    const PosBuf P0 = h->P(h->pc);
+    if (n > 0) {
+        if (s.dim == 3)
+            k_bin_keys<3><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(P0.x, P0.y, P0.z, n, s.rmax, gr.nx, h->keys0, h->cell_count);
+        else
+            k_bin_keys<2><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(P0.x, P0.y, P0.z, n, s.rmax, gr.nx, h->keys0, h->cell_count);
+    }
+    LAUNCH_CHECK();
+    if ((rc = scan_u32(h, h->cell_count, nc, h->cell_start, h->cell_end))) return rc;
+    if ((rc = record(h, ev, 1))) return rc;
+
+    h->stepped = true;
+    h->last_nx = gr.nx;
+    h->last_ncells = gr.ncells;
+    if (n == 0) {
+        h->perm = h->vA;
+        for (int k = 2; k < 5; ++k)
+            if ((rc = record(h, ev, k))) return rc;
+        return PLIFE_OK;
+    }
+
+    // 2. stable sort of (key, index) = the reference's counting-sort permutation
+    if ((rc = radix_sort(h, n, gr.ncells))) return rc;
+    if ((rc = record(h, ev, 2))) return rc;
+
+    // 3. gather into cell order
+    const int po = 1 - h->pc, ao = 1 - h->ac;
+    if (s.dim == 3)
+        k_gather<3><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(n, h->perm, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao));
+    else
+        k_gather<2><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(n, h->perm, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao));
+    LAUNCH_CHECK();
+    if ((rc = record(h, ev, 3))) return rc;
+
+    // 4. force + integrate (Physics.java:122-131, fused: Jacobi)
+    const double R = s.rmax * kFix;
+    const double R2 = R * R;
+    ForceArgs a{};
+    a.sx = h->px[po];
+    a.sy = h->py[po];
+    a.sz = h->pz[po];
+    a.vx = h->vx[ao];
+    a.vy = h->vy[ao];
+    a.vz = h->vz[ao];
+    a.type = h->type[ao];
+    a.id = h->id[ao];
+    a.ox = h->px[h->pc];
+    a.oy = h->py[h->pc];
+    a.oz = h->pz[h->pc];
+    a.cell_start = h->cell_start;
+    a.cell_end = h->cell_end;
+    a.matrix = h->d_matrix;
+    a.ntypes = h->ntypes;
+    a.n = n;
+    a.nx = gr.nx;
+    const double avg = (double)n / (double)gr.ncells;
+    int W = (int)std::lround(kBlock / std::max(avg, 1e-9));
+    W = std::max(1, std::min(W, 64));
+    W = std::min(W, gr.nx);
+    const int NR = s.dim == 3 ? 9 : 3;
+    const int nrows = s.dim == 3 ? gr.nx * gr.nx : gr.nx;
+    a.W = W;
+    a.nstrips = ceil_div(gr.nx, W);
+    const bool m_in_lds = h->ntypes <= kMaxTypesLds;
+    a.m_in_lds = m_in_lds ? 1 : 0;
+    a.lcap = kListCap;
+    const size_t fixed = (m_in_lds ? (size_t)h->ntypes * h->ntypes * 8 : 0) +
+                         (size_t)kListCap * kBlock * 2 + (size_t)(NR * (W + 2) + 1) * 4 +
+                         (size_t)(NR * (W + 2)) * 4;
+    const size_t per = 16 + ((s.dim == 3 && h->debug) ? 4 : 0);
+    long long want = (long long)std::ceil(NR * (W + 2) * avg * 1.5) + 256;
+    want = (want + 63) / 64 * 64;
+    const long long maxcap = ((long long)kMaxLds - 64 - (long long)fixed) / (long long)per;
+    a.cap = (int)std::max(0LL, std::min(want, maxcap));
+    a.all_global = gr.nx < 3 ? 1 : 0;
+    a.r2hi = (float)(R2 * (1.0 + 0x1p-20));
+    a.r2 = s.rmax * s.rmax;                             // Physics.java:432
+    a.rmax = s.rmax;
+    a.inv_rmax = 1.0 / s.rmax;
+    a.beta = s.beta;
+    a.one_plus_beta = 1.0 + s.beta;
+    a.inv_1mb = 1.0 / (1.0 - s.beta);
+    a.inv_beta = 1.0 / s.beta;
+    a.ff = std::pow(s.friction, 60 * s.dt);             // Physics.java:401
+    a.kdt = s.force * s.dt;
+    a.cs = s.rmax;
+    a.dt = s.dt;
+    a.pair_counter = h->d_pairs;
+    a.dbg_count = h->dbg_count;
+    a.dbg_hash = h->dbg_hash;
+    const size_t lds = fixed + (size_t)a.cap * per;
+    const long long grid = (long long)a.nstrips * nrows;
+    if (grid > 0x7fffffffLL) return fail(PLIFE_ERR_ARGUMENT, "grid of %lld workgroups too large", grid);
+    if ((rc = launch_force_any(h, a, (int)grid, lds))) return rc;
+    if ((rc = record(h, ev, 4))) return rc;
+    h->ac = ao;
+    return PLIFE_OK;
+}
+
+int check_ready(plife_physics *h)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!h->have_settings) return fail(PLIFE_ERR_STATE, "plife_set_settings has not been called");
+    return PLIFE_OK;
+}
+
+uint32_t to_fixed(double x, bool wrap)
+{
+    // Physics.ensurePosition (Physics.java:499-505), then round to 2^-32.
+    if (!(x == x)) return 0u;
+    if (wrap) {
+        x = x - std::floor(x);
+        double s = std::rint(x * kFix);
+        return (uint32_t)(unsigned long long)s;
+    }
+    x = x < 0 ? 0 : (x > 1 ? 1 : x);
+    double s = std::rint(x * kFix);
+    return s >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(unsigned long long)s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *plife_last_error(void) { return g_last_error.c_str(); }
+
+int plife_create(int32_t device, plife_physics **out)
+{
+    if (!out) return fail(PLIFE_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(PLIFE_ERR_ARGUMENT, "device %d out of range (%d devices)", device, count);
+    plife_physics *h = new plife_physics();
+    h->device = device;
+    int rc = set_device(h);
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = dev_alloc(&h->d_pairs, 1);
+    if (!rc) rc = dev_alloc(&h->d_matrix, kMaxTypes * kMaxTypes);
+    if (rc) {
+        plife_destroy(h);
+        return rc;
+    }
+    *out = h;
+    g_last_error.clear();
+    return PLIFE_OK;
+}
+
+int plife_destroy(plife_physics *h)
+{
+    if (!h) return PLIFE_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_state(h);
+    dev_free(h->cell_count);
+    dev_free(h->cell_start);
+    dev_free(h->cell_end);
+    dev_free(h->hist);
+    dev_free(h->bsum);
+    dev_free(h->d_pairs);
+    dev_free(h->d_matrix);
+    dev_free(h->dbg_count);
+    dev_free(h->dbg_hash);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PLIFE_OK;
+}
+
+int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
+                       const double *matrix)
+{
+    if (!h || !s || !matrix) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (s->dim != 2 && s->dim != 3) return fail(PLIFE_ERR_ARGUMENT, "dim must be 2 or 3, got %d", s->dim);
+    if (!(s->rmax > 0.0) || !(s->rmax <= 1.0))
+        return fail(PLIFE_ERR_ARGUMENT, "rmax must be in (0, 1], got %g", s->rmax);
+    if (!(s->beta > 0.0) || !(s->beta < 1.0))
+        return fail(PLIFE_ERR_ARGUMENT, "beta must be in (0, 1), got %g", s->beta);
+    if (!std::isfinite(s->dt) || !std::isfinite(s->force) || !std::isfinite(s->friction))
+        return fail(PLIFE_ERR_ARGUMENT, "dt, force and friction must be finite");
+    if (ntypes < 1 || ntypes > kMaxTypes)
+        return fail(PLIFE_ERR_ARGUMENT, "matrix size must be in [1, %d], got %d", kMaxTypes, ntypes);
+    const Grid g = grid_of(*s);
+    if (g.ncells > kMaxCells)
+        return fail(PLIFE_ERR_ARGUMENT, "rmax %g gives %lld cells (limit %lld)", s->rmax, g.ncells, kMaxCells);
+    int rc = set_device(h);
+    if (rc) return rc;
+    const size_t mm = (size_t)ntypes * ntypes;
+    const bool changed = ntypes != h->ntypes || h->matrix_host.size() != mm ||
+                         std::memcmp(h->matrix_host.data(), matrix, mm * sizeof(double)) != 0;
+    if (changed) {
+        h->matrix_host.assign(matrix, matrix + mm);
+        HIP_TRY(hipMemcpyAsync(h->d_matrix, h->matrix_host.data(), mm * sizeof(double),
+                               hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->ntypes = ntypes;
+    h->st = *s;
+    h->have_settings = true;
+    return PLIFE_OK;
+}
+
+int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const double *vel,
+                        const int32_t *types)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (n < 0) return fail(PLIFE_ERR_ARGUMENT, "n must be >= 0, got %d", n);
+    if (n > 0 && (!pos || !vel || !types)) return fail(PLIFE_ERR_ARGUMENT, "null array");
+    for (int i = 0; i < n; ++i)
+        if (types[i] < 0 || types[i] >= h->ntypes)
+            return fail(PLIFE_ERR_ARGUMENT, "particle %d has type %d outside [0, %d)", i, types[i], h->ntypes);
+    if ((rc = set_device(h))) return rc;
+    if ((rc = ensure_capacity(h, n))) return rc;
+    const bool wrap = h->st.wrap != 0;
+    const bool d3 = h->st.dim == 3;
+    std::vector<uint32_t> x(n), y(n), z(n);
+    std::vector<float> fx(n), fy(n), fz(n);
+    std::vector<int32_t> ids(n);
+    for (int i = 0; i < n; ++i) {
+        x[i] = to_fixed(pos[3 * i], wrap);
+        y[i] = to_fixed(pos[3 * i + 1], wrap);
+        z[i] = d3 ? to_fixed(pos[3 * i + 2], wrap) : 0u;
+        fx[i] = (float)vel[3 * i];
+        fy[i] = (float)vel[3 * i + 1];
+        fz[i] = d3 ? (float)vel[3 * i + 2] : 0.0f;
+        ids[i] = i;
+    }
+    h->pc = 0;
+    h->ac = 0;
+    h->n = n;
+    h->stepped = false;
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(h->px[0], x.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->py[0], y.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->pz[0], z.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->vx[0], fx.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->vy[0], fy.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->vz[0], fz.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->type[0], types, n * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->id[0], ids.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PLIFE_OK;
+}
+
+int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
+                        int32_t *ids)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (n != h->n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but the handle holds %d particles", n, h->n);
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (n == 0) return PLIFE_OK;
+    const PosBuf P = h->P(h->pc);
+    const AuxBuf A = h->A(h->ac);
+    const bool d3 = h->have_settings && h->st.dim == 3;
+    if (pos) {
+        std::vector<uint32_t> x(n), y(n), z(n);
+        HIP_TRY(hipMemcpy(x.data(), P.x, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(y.data(), P.y, n * 4, hipMemcpyDeviceToHost));
+        if (d3) HIP_TRY(hipMemcpy(z.data(), P.z, n * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            pos[3 * i] = (double)x[i] * kInvFix;
+            pos[3 * i + 1] = (double)y[i] * kInvFix;
+            pos[3 * i + 2] = d3 ? (double)z[i] * kInvFix : 0.0;
+        }
+    }
+    if (vel) {
+        std::vector<float> x(n), y(n), z(n);
+        HIP_TRY(hipMemcpy(x.data(), A.vx, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(y.data(), A.vy, n * 4, hipMemcpyDeviceToHost));
+        if (d3) HIP_TRY(hipMemcpy(z.data(), A.vz, n * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            vel[3 * i] = x[i];
+            vel[3 * i + 1] = y[i];
+            vel[3 * i + 2] = d3 ? (double)z[i] : 0.0;
+        }
+    }
+    if (types) HIP_TRY(hipMemcpy(types, A.type, n * 4, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(hipMemcpy(ids, A.id, n * 4, hipMemcpyDeviceToHost));
+    return PLIFE_OK;
+}
+
+int plife_get_count(plife_physics *h, int32_t *n)
+{
+    if (!h || !n) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    *n = h->n;
+    return PLIFE_OK;
+}
+
+int plife_update(plife_physics *h, int32_t nsteps)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "nsteps must be >= 0");
+    if ((rc = set_device(h))) return rc;
+    h->stop.store(0);
+    for (int k = 0; k < nsteps; ++k) {
+        if (h->stop.load()) return PLIFE_STOPPED;
+        if ((rc = do_step(h, nullptr))) return rc;
+    }
+    return PLIFE_OK;
+}
+
+int plife_force_update_stop(plife_physics *h)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    h->stop.store(1);
+    return PLIFE_OK;
+}
+
+int plife_synchronize(plife_physics *h)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PLIFE_OK;
+}
+
+int plife_get_grid(plife_physics *h, int32_t *nx, int32_t *ncells)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    const Grid g = grid_of(h->st);
+    if (nx) *nx = g.nx;
+    if (ncells) *ncells = (int32_t)g.ncells;
+    return PLIFE_OK;
+}
+
+int plife_get_cells(plife_physics *h, int32_t *keys, int32_t *cell_end, int32_t *perm,
+                    int32_t ncells)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!h->stepped) return fail(PLIFE_ERR_STATE, "no step has run since the state was set");
+    if (ncells != h->last_ncells)
+        return fail(PLIFE_ERR_ARGUMENT, "ncells = %d but the last step used %lld cells", ncells, h->last_ncells);
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (keys && h->n) HIP_TRY(hipMemcpy(keys, h->keys0, (size_t)h->n * 4, hipMemcpyDeviceToHost));
+    if (cell_end) HIP_TRY(hipMemcpy(cell_end, h->cell_end, (size_t)ncells * 4, hipMemcpyDeviceToHost));
+    if (perm && h->n) HIP_TRY(hipMemcpy(perm, h->perm, (size_t)h->n * 4, hipMemcpyDeviceToHost));
+    return PLIFE_OK;
+}
+
+int plife_get_pair_count(plife_physics *h, int64_t *pairs)
+{
+    if (!h || !pairs) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    int rc = set_device(h);
+    if (rc) return rc;
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, h->d_pairs, 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *pairs = (int64_t)v;
+    return PLIFE_OK;
+}
+
+int plife_set_debug(plife_physics *h, int32_t on)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    h->debug = on ? 1 : 0;
+    return PLIFE_OK;
+}
+
+int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!h->debug || !h->stepped) return fail(PLIFE_ERR_STATE, "debug mode was off or no step has run");
+    if (n != h->n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but the handle holds %d particles", n, h->n);
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (n == 0) return PLIFE_OK;
+    if (count) HIP_TRY(hipMemcpy(count, h->dbg_count, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (hash) HIP_TRY(hipMemcpy(hash, h->dbg_hash, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return PLIFE_OK;
+}
+
+int plife_time_steps(plife_physics *h, int32_t nsteps, double *ms)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!ms || nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
+    if ((rc = set_device(h))) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    for (int k = 0; k < nsteps && !rc; ++k) rc = do_step(h, nullptr);
+    if (!rc) {
+        HIP_TRY(hipEventRecord(e1, h->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!out || nsteps <= 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
+    if ((rc = set_device(h))) return rc;
+    std::vector<PhaseEvents> ev(nsteps);
+    for (auto &pe : ev)
+        for (auto &e : pe.e) HIP_TRY(hipEventCreate(&e));
+    for (int k = 0; k < nsteps && !rc; ++k) rc = do_step(h, &ev[k]);
+    if (!rc) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        double sum[4] = {0, 0, 0, 0};
+        for (auto &pe : ev)
+            for (int p = 0; p < 4; ++p) {
+                float f = 0;
+                HIP_TRY(hipEventElapsedTime(&f, pe.e[p], pe.e[p + 1]));
+                sum[p] += f;
+            }
+        float wall = 0;
+        HIP_TRY(hipEventElapsedTime(&wall, ev.front().e[0], ev.back().e[4]));
+        out->bin_ms = sum[0] / nsteps;
+        out->sort_ms = sum[1] / nsteps;
+        out->gather_ms = sum[2] / nsteps;
+        out->force_ms = sum[3] / nsteps;
+        out->step_ms = wall / nsteps;
+        out->steps = nsteps;
+        int64_t pairs = 0;
+        rc = plife_get_pair_count(h, &pairs);
+        out->pairs = pairs;
+    }
+    for (auto &pe : ev)
+        for (auto &e : pe.e) (void)hipEventDestroy(e);
+    return rc;
+}
+
+}  // extern "C"

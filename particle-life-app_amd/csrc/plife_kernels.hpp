@@ -1,0 +1,347 @@
+// plife_kernels.hpp -- gfx950 kernels of the particle-life physics step.
+//
+// One step (reference: backend/Physics.java:112-134):
+//   k_fill / k_bin_keys      cell key per particle (Physics.java:362-375) + cell histogram
+//   k_scan_*                 exclusive/inclusive scan -> cell_start / cell_end
+//                            (cell_end is the reference's `containers` array, Physics.java:329-348)
+//   k_radix_hist/_scatter    stable LSD radix sort of (key, index): the reference's
+//                            stable counting-sort permutation (Physics.java:342-353)
+//   k_gather                 SoA reorder into cell order (the array swap, Physics.java:350-353)
+//   k_force (plife_force.hpp) velocity phase + position phase fused (Physics.java:397-450);
+//                            legal because the update is Jacobi: velocities read only the
+//                            sorted step-start positions, new positions go to another buffer.
+//
+// Positions are 32-bit fixed point (x = k * 2^-32, unit box), so the periodic
+// minimum image of Range.wrapConnection (Range.java:68-81) is the exact int32
+// difference k_q - k_p, and converting a position to FP64 is exact.  Cell keys
+// use the same FP64 division and truncation as the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plife {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr double kFix = 4294967296.0;       // 2^32
+constexpr double kInvFix = 1.0 / 4294967296.0;
+
+// ---------------------------------------------------------------------------
+// wave64 / block helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block.  s_w: 4 words of LDS.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, uint32_t &total)
+{
+    const uint32_t inc = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) {
+        uint32_t t = s_w[k];
+        off += (k < w) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+// Sum over a 256-thread block; every thread gets the total.  s_w: 4 words of LDS.
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v,
+                                                            unsigned long long *s_w)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) s_w[w] = v;
+    __syncthreads();
+    unsigned long long tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) tot += s_w[k];
+    __syncthreads();
+    return tot;
+}
+
+// Block ids are dealt round-robin to the 8 XCDs; remap so each XCD works on a
+// contiguous run of tiles and neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int q = nb >> 3, rem = nb & 7;
+    const int xcd = b & 7, k = b >> 3;
+    return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+}
+
+// ---------------------------------------------------------------------------
+// fill
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// cell keys (Physics.java:362-375): cx = trunc(x / cs) in FP64, clamped to nx-1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int key_axis(uint32_t k, double cs, int nx)
+{
+    const double x = (double)k * kInvFix;   // exact
+    int c = (int)(x / cs);                   // IEEE FP64 division, x >= 0: trunc == floor
+    return c == nx ? nx - 1 : c;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_bin_keys(const uint32_t *__restrict__ px,
+                                                     const uint32_t *__restrict__ py,
+                                                     const uint32_t *__restrict__ pz, int n,
+                                                     double cs, int nx,
+                                                     uint32_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ cell_count)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
+    if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
+    keys[i] = key;
+    atomicAdd(&cell_count[key], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// device-wide scan of u32 (reduce -> scan block sums -> downsweep)
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in, int n,
+                                                        uint32_t *__restrict__ bsum)
+{
+    __shared__ uint32_t s_w[4];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t total;
+    block_excl_scan(s, s_w, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// In-place exclusive scan of a[0..n) by one block (n = number of tiles).
+__global__ __launch_bounds__(kBlock) void k_scan_single(uint32_t *__restrict__ a, int n)
+{
+    __shared__ uint32_t s_w[4];
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += kScanTile) {
+        const int i0 = base + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (i0 + k < n) ? a[i0 + k] : 0u;
+            s += v[k];
+        }
+        uint32_t total;
+        uint32_t off = block_excl_scan(s, s_w, total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < n) a[i0 + k] = off;
+            off += v[k];
+        }
+        carry += total;
+    }
+}
+
+// `in` may alias `excl` (in-place scan): no __restrict__ on those two.
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, int n,
+                                                      const uint32_t *__restrict__ boff,
+                                                      uint32_t *excl,
+                                                      uint32_t *__restrict__ incl)
+{
+    __shared__ uint32_t s_w[4];
+    const int i0 = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(s, s_w, total) + boff[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (i0 + k < n) {
+            if (excl) excl[i0 + k] = off;
+            off += v[k];
+            if (incl) incl[i0 + k] = off;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (key, value) -- reduce-then-scan per digit pass
+// ---------------------------------------------------------------------------
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kBlock * kRadixItems;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys, int n,
+                                                       int shift, int ntiles,
+                                                       uint32_t *__restrict__ hist)
+{
+    constexpr int R = 1 << D;
+    __shared__ uint32_t h[R];
+    for (int t = threadIdx.x; t < R; t += kBlock) h[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRadixTile;
+#pragma unroll 4
+    for (int k = 0; k < kRadixItems; ++k) {
+        const int i = base + k * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R; t += kBlock) hist[t * ntiles + blockIdx.x] = h[t];
+}
+
+// Stable scatter.  Elements are ranked in index order: round r covers indices
+// base + r*256 + tid, and inside a round waves go in order and lanes go in
+// order (wave-level match by digit ballots).  gofs is the digit-major
+// exclusive scan of the per-tile histograms.  vin == nullptr means value = index.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                          const uint32_t *__restrict__ vin,
+                                                          uint32_t *__restrict__ kout,
+                                                          uint32_t *__restrict__ vout,
+                                                          const uint32_t *__restrict__ gofs,
+                                                          int n, int shift, int ntiles)
+{
+    constexpr int R = 1 << D;
+    constexpr int NW = kBlock / kWave;
+    __shared__ uint32_t s_k[kRadixTile];
+    __shared__ uint32_t s_v[kRadixTile];
+    __shared__ uint32_t s_cnt[NW][R];
+    __shared__ uint32_t s_off[NW][R];
+    __shared__ uint32_t s_start[R];
+    __shared__ uint32_t s_gb[R];
+    __shared__ uint32_t s_w[4];
+
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int base = blockIdx.x * kRadixTile;
+    for (int t = tid; t < NW * R; t += kBlock) (&s_cnt[0][0])[t] = 0;
+    uint32_t running = 0;
+    uint32_t key[kRadixItems], val[kRadixItems], rank[kRadixItems];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        const int i = base + r * kBlock + tid;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u;
+        const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+        const uint32_t d = (k >> shift) & (R - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t lr = (uint32_t)__popcll(peers & lt);
+        if (valid && lr == 0) s_cnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (tid < R) {
+            uint32_t o = running;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) {
+                s_off[ww][tid] = o;
+                o += s_cnt[ww][tid];
+                s_cnt[ww][tid] = 0;
+            }
+            running = o;
+        }
+        __syncthreads();
+        key[r] = k;
+        val[r] = v;
+        rank[r] = valid ? s_off[w][d] + lr : 0u;
+    }
+
+    uint32_t total;
+    const uint32_t st = block_excl_scan(tid < R ? running : 0u, s_w, total);
+    if (tid < R) {
+        s_start[tid] = st;
+        s_gb[tid] = gofs[tid * ntiles + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        const int i = base + r * kBlock + tid;
+        if (i < n) {
+            const uint32_t d = (key[r] >> shift) & (R - 1);
+            const uint32_t lp = s_start[d] + rank[r];
+            s_k[lp] = key[r];
+            s_v[lp] = val[r];
+        }
+    }
+    __syncthreads();
+    const int nvalid = min(kRadixTile, n - base);
+    for (int j = tid; j < nvalid; j += kBlock) {
+        const uint32_t k = s_k[j];
+        const uint32_t d = (k >> shift) & (R - 1);
+        const uint32_t g = s_gb[d] + (uint32_t)j - s_start[d];
+        if (kout) kout[g] = k;
+        vout[g] = s_v[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SoA state
+// ---------------------------------------------------------------------------
+struct PosBuf {
+    uint32_t *x, *y, *z;
+};
+struct AuxBuf {
+    float *vx, *vy, *vz;
+    int32_t *type, *id;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_gather(int n, const uint32_t *__restrict__ perm,
+                                                   PosBuf ps, AuxBuf as, PosBuf pd, AuxBuf ad)
+{
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    pd.x[s] = ps.x[o];
+    pd.y[s] = ps.y[o];
+    ad.vx[s] = as.vx[o];
+    ad.vy[s] = as.vy[o];
+    if constexpr (DIM == 3) {
+        pd.z[s] = ps.z[o];
+        ad.vz[s] = as.vz[o];
+    }
+    ad.type[s] = as.type[o];
+    ad.id[s] = as.id[o];
+}
+
+}  // namespace plife

@@ -1,0 +1,18 @@
+"""particle_life -- MI355X (gfx950) physics step of particle-life.
+
+Host-side mirror of the reference's backend API (physics.py), the C-ABI
+binding (engine.py, _native.py) and seeded benchmark worlds (worlds.py).
+"""
+from .engine import Engine
+from .physics import (Accelerator, DefaultMatrix, DefaultMatrixGenerator, DefaultPositionSetter,
+                      DefaultTypeSetter, GpuPhysics, Matrix, MatrixGenerator, Particle, Particles,
+                      ParticleLifeAccelerator, PhysicsSettings, PhysicsSnapshot, PositionSetter,
+                      TypeSetter)
+from . import worlds
+
+__all__ = [
+    "Accelerator", "DefaultMatrix", "DefaultMatrixGenerator", "DefaultPositionSetter",
+    "DefaultTypeSetter", "Engine", "GpuPhysics", "Matrix", "MatrixGenerator", "Particle",
+    "Particles", "ParticleLifeAccelerator", "PhysicsSettings", "PhysicsSnapshot",
+    "PositionSetter", "TypeSetter", "worlds",
+]

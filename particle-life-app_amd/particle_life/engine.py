@@ -1,0 +1,127 @@
+"""Thin object wrapper of one libplife handle (include/plife.h).
+
+``Engine`` is the direct, array-in/array-out form of the C ABI used by the
+tests and the benchmark; ``GpuPhysics`` (physics.py) layers the reference's
+Physics API on top of it.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from ._native import PlifeSettings, PlifeTiming, check
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._lib = _native.load()
+        h = ctypes.c_void_p()
+        check(self._lib.plife_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.n = 0
+
+    # lifecycle
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.plife_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # configuration and state
+    def configure(self, *, matrix, rmax: float, friction: float, dt: float, beta: float = 0.3,
+                  force: float = 1.0, wrap: bool = True, dim: int = 2) -> None:
+        m = np.ascontiguousarray(matrix, dtype=np.float64)
+        s = PlifeSettings(int(dim), 1 if wrap else 0, float(rmax), float(friction), float(force),
+                          float(dt), float(beta))
+        check(self._lib.plife_set_settings(self._h, ctypes.byref(s), int(m.shape[0]), m.ctypes.data))
+
+    def configure_world(self, w) -> None:
+        self.configure(matrix=w.matrix, rmax=w.rmax, friction=w.friction, dt=w.dt, beta=w.beta,
+                       force=w.force, wrap=w.wrap, dim=w.dim)
+
+    def set_state(self, pos, vel, types) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        n = int(types.shape[0])
+        check(self._lib.plife_set_particles(self._h, n, pos.ctypes.data, vel.ctypes.data,
+                                            types.ctypes.data))
+        self.n = n
+
+    def load_world(self, w) -> None:
+        self.configure_world(w)
+        self.set_state(w.pos, w.vel, w.types)
+
+    def state(self) -> dict:
+        n = self.n
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        typ = np.zeros(n, np.int32)
+        ids = np.zeros(n, np.int32)
+        check(self._lib.plife_get_particles(self._h, n, pos.ctypes.data, vel.ctypes.data,
+                                            typ.ctypes.data, ids.ctypes.data))
+        return {"pos": pos, "vel": vel, "types": typ, "ids": ids}
+
+    # stepping
+    def step(self, nsteps: int = 1) -> int:
+        return check(self._lib.plife_update(self._h, int(nsteps)))
+
+    def stop(self) -> None:
+        check(self._lib.plife_force_update_stop(self._h))
+
+    def sync(self) -> None:
+        check(self._lib.plife_synchronize(self._h))
+
+    # parity exports
+    def grid(self) -> tuple[int, int]:
+        nx, nc = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.plife_get_grid(self._h, ctypes.byref(nx), ctypes.byref(nc)))
+        return nx.value, nc.value
+
+    def cells(self) -> dict:
+        _, nc = self.grid()
+        keys = np.zeros(self.n, np.int32)
+        cell_end = np.zeros(nc, np.int32)
+        perm = np.zeros(self.n, np.int32)
+        check(self._lib.plife_get_cells(self._h, keys.ctypes.data, cell_end.ctypes.data,
+                                        perm.ctypes.data, nc))
+        return {"keys": keys, "cell_end": cell_end, "perm": perm}
+
+    def pair_count(self) -> int:
+        v = ctypes.c_int64()
+        check(self._lib.plife_get_pair_count(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    def set_debug(self, on: bool) -> None:
+        check(self._lib.plife_set_debug(self._h, 1 if on else 0))
+
+    def neighbor_summary(self) -> tuple[np.ndarray, np.ndarray]:
+        count = np.zeros(self.n, np.int32)
+        h = np.zeros(self.n, np.uint64)
+        check(self._lib.plife_get_neighbor_summary(self._h, self.n, count.ctypes.data, h.ctypes.data))
+        return count, h
+
+    # timing
+    def time_steps(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        check(self._lib.plife_time_steps(self._h, int(nsteps), ctypes.byref(ms)))
+        return float(ms.value)
+
+    def profile(self, nsteps: int) -> dict:
+        t = PlifeTiming()
+        check(self._lib.plife_profile(self._h, int(nsteps), ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in PlifeTiming._fields_}

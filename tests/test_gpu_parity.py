@@ -1,0 +1,165 @@
+"""Device step vs the FP64 oracle on identical states (see parity_util for the
+tolerances).  Every case prints its name first so a failing GPU run names it."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from parity_util import check_one_step, free_run_error
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from particle_life import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _world(seed, n, ntypes, rmax, dt, **kw):
+    from particle_life import worlds
+
+    return worlds.uniform_world(seed, n, ntypes, rmax=rmax, dt=dt, **kw)
+
+
+CASES = {
+    # name: (seed, n, types, rmax, dt, extra)
+    "c1_10k_2d": (101, 10_000, 6, 0.02, 0.02, {}),
+    "clamped_2d": (102, 6_000, 5, 0.03, 0.02, {"wrap": False}),
+    "noninteger_cells_2d": (103, 8_000, 7, 0.07, 0.02, {}),          # last cell wider (Q2)
+    "nx3_2d": (104, 2_000, 4, 0.30, 0.02, {}),
+    "nx2_repeated_cells_2d": (105, 400, 3, 0.40, 0.02, {}),          # Q1: repeated stencil cells
+    "nx1_2d": (106, 50, 3, 0.9, 0.02, {}),
+    "strong_force_2d": (107, 5_000, 8, 0.025, 0.05, {"force": 40.0, "beta": 0.2}),
+    "dense_2d": (108, 40_000, 6, 0.05, 0.01, {}),                    # 100/cell: wide strips
+    "3d": (109, 20_000, 6, 0.06, 0.01, {"dim": 3}),
+    "3d_clamped": (110, 8_000, 5, 0.09, 0.01, {"dim": 3, "wrap": False}),
+    "3d_nx2": (111, 300, 3, 0.45, 0.01, {"dim": 3}),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_one_step_parity(eng, name):
+    seed, n, t, rmax, dt, kw = CASES[name]
+    print(f"\n[parity] case={name} n={n} rmax={rmax}", flush=True)
+    w = _world(seed, n, t, rmax, dt, **kw)
+    eng.load_world(w)
+    for k in range(3):
+        st = check_one_step(eng, w)
+        print(f"  step {k}: {st}", flush=True)
+
+
+def test_ten_steps_each_checked(eng):
+    """Ten consecutive steps, each compared with the oracle stepping the device state."""
+    from particle_life import worlds
+
+    w = worlds.config1()
+    print("\n[parity] case=c1_ten_steps", flush=True)
+    eng.load_world(w)
+    for k in range(10):
+        check_one_step(eng, w, debug=(k % 3 == 0))
+
+
+def test_trajectory_10_steps(eng):
+    """Free-running 10-step trajectory stays within 1e-8 (unit box) of the oracle's."""
+    from particle_life import worlds
+
+    w = worlds.config1()
+    print("\n[parity] case=c1_trajectory", flush=True)
+    err = free_run_error(eng, w, 10)
+    print(f"  max position error after 10 steps: {err:.3e}")
+    assert err < 1e-8
+
+
+def test_heavy_cells_global_path(eng):
+    """A blob dense enough that staged strips overflow LDS and take the global
+    path, mixed with sparse strips on the LDS path."""
+    from particle_life import worlds
+
+    print("\n[parity] case=heavy_blob_2d", flush=True)
+    w = worlds.gaussian_blob_world(112, 20_000, 6, rmax=0.02, dt=0.01, beta=0.3, nblobs=2,
+                                   sigma=0.02, dim=2)
+    eng.load_world(w)
+    for _ in range(2):
+        check_one_step(eng, w)
+
+
+def test_clustered_3d(eng):
+    from particle_life import worlds
+
+    print("\n[parity] case=clustered_3d", flush=True)
+    w = worlds.gaussian_blob_world(113, 40_000, 12, rmax=0.04, dt=0.01, beta=0.25, nblobs=8,
+                                   sigma=0.03, dim=3)
+    eng.load_world(w)
+    check_one_step(eng, w)
+
+
+def test_edge_sizes(eng):
+    """Empty world, one particle, two particles, coincident particles."""
+    from particle_life import worlds
+
+    print("\n[parity] case=edge_sizes", flush=True)
+    base = _world(114, 3, 3, 0.05, 0.02)
+    for n in (0, 1, 2):
+        w = worlds.World(pos=base.pos[:n], vel=base.vel[:n], types=base.types[:n],
+                         matrix=base.matrix, rmax=0.05, friction=base.friction, dt=0.02)
+        eng.load_world(w)
+        check_one_step(eng, w)
+    # two coincident particles plus one within range: d^2 == 0 is excluded
+    pos = np.array([[0.5, 0.5, 0], [0.5, 0.5, 0], [0.52, 0.5, 0]])
+    w = worlds.World(pos=pos, vel=np.zeros((3, 3)), types=np.array([0, 1, 2], np.int32),
+                     matrix=base.matrix, rmax=0.05, friction=base.friction, dt=0.02)
+    eng.load_world(w)
+    st = check_one_step(eng, w)
+    assert st["pairs"] == 4
+
+
+def test_seam_and_exact_distance(eng):
+    """Pairs across the periodic seam and at exactly d = rmax (the predicate
+    is <=, the force there is zero)."""
+    from particle_life import worlds
+
+    print("\n[parity] case=seam_and_ties", flush=True)
+    r = 2.0 ** -6
+    pos = np.array([
+        [2.0 ** -20, 0.5, 0], [1 - 2.0 ** -12, 0.5, 0],       # across x seam
+        [0.25, 2.0 ** -24, 0], [0.25, 1 - 2.0 ** -10, 0],      # across y seam
+        [0.6, 0.6, 0], [0.6 + r, 0.6, 0],                      # exactly rmax apart
+        [0.3, 0.3, 0], [0.3 + r, 0.3 + 2.0 ** -32, 0],         # just beyond
+    ])
+    n = len(pos)
+    w = worlds.World(pos=pos, vel=np.zeros((n, 3)), types=np.arange(n, dtype=np.int32) % 3,
+                     matrix=np.array([[0.5, -1, 1], [1, 0.25, -0.5], [-1, 1, 0.75]]),
+                     rmax=r, friction=0.9, dt=0.02)
+    eng.load_world(w)
+    st = check_one_step(eng, w)
+    assert st["pairs"] == 6
+
+
+def test_dt_and_matrix_change_between_steps(eng):
+    """Settings are read live each step (Physics.java:401-437; app/Main.java:292-295)."""
+    w = _world(115, 5_000, 6, 0.03, 0.02)
+    print("\n[parity] case=live_settings", flush=True)
+    eng.load_world(w)
+    check_one_step(eng, w)
+    w.dt = 0.013
+    w.matrix = -w.matrix
+    w.rmax = 0.041
+    eng.configure_world(w)
+    check_one_step(eng, w)
+
+
+@pytest.mark.parametrize("which", ["c2"])
+def test_full_size_c2_one_step(eng, which):
+    """The 1M-particle 2D world at full size: one step checked exactly."""
+    from particle_life import worlds
+
+    print("\n[parity] case=c2_full_size", flush=True)
+    w = worlds.config2()
+    eng.load_world(w)
+    st = check_one_step(eng, w, nthreads=16, debug=True)
+    print(f"  {st}")
+    assert 70 * w.n < st["pairs"] < 90 * w.n
