@@ -69,7 +69,8 @@ int plife_destroy(plife_physics *h);
 /* Reads PhysicsSettings + Matrix (backend/Matrix.java:3-8, DefaultMatrix.java:39-41,
  * row = type of the acted-on particle) + beta.  Cheap to call every frame
  * (app/Main.java:292-295 overwrites dt per frame); the matrix is re-uploaded
- * only when it changed.  ntypes in [1, 64]. */
+ * only when it changed.  ntypes in [1, 256] (the app caps it at 256,
+ * app/Main.java:745). */
 int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
                        const double *matrix);
 
@@ -121,6 +122,22 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs);
  * plife_get_neighbor_summary downloads them.  Off by default. */
 int plife_set_debug(plife_physics *h, int32_t on);
 int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash);
+
+/* Launch shape of the force kernel (no counterpart in the reference, which has
+ * no device).  Zero fields mean automatic.  strip_cells in [0, 64]: home cells
+ * per workgroup strip; list_cap in [0, 512]: per-thread LDS candidate-list
+ * entries; stage_slack in [0, 16]: LDS staging capacity as a multiple of the
+ * expected staged particles.  Results do not depend on the tuning. */
+typedef struct {
+    int32_t strip_cells;
+    int32_t list_cap;
+    double stage_slack;
+} plife_tuning;
+
+int plife_set_tuning(plife_physics *h, const plife_tuning *t);
+/* Current tuning with automatic fields resolved from the last step, and the
+ * LDS bytes per workgroup of the last force launch. */
+int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes);
 
 /* Runs nsteps steps with HIP events around each phase and fills *out. */
 int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out);

@@ -51,7 +51,8 @@ int fail(int code, const char *fmt, ...)
 
 constexpr int kMaxTypes = 256;        // the app caps the matrix size at 256 (app/Main.java:745)
 constexpr int kMaxTypesLds = 32;      // matrix staged in LDS up to 32x32 (8 KB of FP64)
-constexpr int kListCap = 64;          // per-thread candidate list entries (u16) in LDS
+constexpr int kListCap = 24;          // per-thread candidate list entries (u16) in LDS
+constexpr double kHomesPerStrip = 200; // target home particles per 256-thread workgroup
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxLds = 65536;
 
@@ -119,6 +120,11 @@ struct plife_physics {
     unsigned long long *dbg_hash = nullptr;
 
     std::atomic<int> stop{0};
+
+    // launch tuning (0 = automatic) and what the last force launch used
+    plife_tuning tune{};
+    size_t last_lds = 0;
+    int last_W = 0, last_cap = 0;
 
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
@@ -397,41 +403,47 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.n = n;
     a.nx = gr.nx;
     const double avg = (double)n / (double)gr.ncells;
-    int W = (int)std::lround(kBlock / std::max(avg, 1e-9));
-    W = std::max(1, std::min(W, 64));
-    W = std::min(W, gr.nx);
     const int NR = s.dim == 3 ? 9 : 3;
     const int nrows = s.dim == 3 ? gr.nx * gr.nx : gr.nx;
+    // launch shape: strips of W cells holding about kHomesPerStrip homes
+    int W = h->tune.strip_cells > 0 ? h->tune.strip_cells
+                                    : (int)std::lround(kHomesPerStrip / std::max(avg, 1e-9));
+    W = std::max(1, std::min(W, 64));
+    W = std::min(W, gr.nx);
     a.W = W;
     a.nstrips = ceil_div(gr.nx, W);
     const bool m_in_lds = h->ntypes <= kMaxTypesLds;
     a.m_in_lds = m_in_lds ? 1 : 0;
-    a.lcap = kListCap;
-    const size_t fixed = (m_in_lds ? (size_t)h->ntypes * h->ntypes * 8 : 0) +
-                         (size_t)kListCap * kBlock * 2 + (size_t)(NR * (W + 2) + 1) * 4 +
-                         (size_t)(NR * (W + 2)) * 4;
-    const size_t per = 16 + ((s.dim == 3 && h->debug) ? 4 : 0);
-    long long want = (long long)std::ceil(NR * (W + 2) * avg * 1.5) + 256;
-    want = (want + 63) / 64 * 64;
-    const long long maxcap = ((long long)kMaxLds - 64 - (long long)fixed) / (long long)per;
-    a.cap = (int)std::max(0LL, std::min(want, maxcap));
+    a.lcap = h->tune.list_cap > 0 ? std::max(8, std::min(h->tune.list_cap, 512)) : kListCap;
+    const double slack = h->tune.stage_slack > 0 ? h->tune.stage_slack : 1.25;
+    long long want = (long long)std::ceil(NR * (W + 2) * avg * slack) + 128;
+    want = std::min<long long>((want + 63) / 64 * 64, 65535);
+    // shrink the staging capacity until the layout fits the LDS budget
+    const bool dbg = h->debug != 0;
+    while (want > 0 && force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, (int)want, a.lcap, W).total >
+                           (size_t)kMaxLds)
+        want -= 64;
+    a.cap = (int)std::max(0LL, want);
     a.all_global = gr.nx < 3 ? 1 : 0;
     a.r2hi = (float)(R2 * (1.0 + 0x1p-20));
-    a.r2 = s.rmax * s.rmax;                             // Physics.java:432
-    a.rmax = s.rmax;
-    a.inv_rmax = 1.0 / s.rmax;
+    a.r2u = (s.rmax * s.rmax) * (kFix * kFix);          // Physics.java:432, exact 2^64 scaling
+    a.S = 1.0 / R;
+    a.R = R;
     a.beta = s.beta;
     a.one_plus_beta = 1.0 + s.beta;
     a.inv_1mb = 1.0 / (1.0 - s.beta);
     a.inv_beta = 1.0 / s.beta;
     a.ff = std::pow(s.friction, 60 * s.dt);             // Physics.java:401
-    a.kdt = s.force * s.dt;
+    a.kdt = s.force * s.dt / kFix;
     a.cs = s.rmax;
     a.dt = s.dt;
     a.pair_counter = h->d_pairs;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
-    const size_t lds = fixed + (size_t)a.cap * per;
+    const size_t lds = force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, a.cap, a.lcap, W).total;
+    h->last_lds = lds;
+    h->last_W = W;
+    h->last_cap = a.cap;
     const long long grid = (long long)a.nstrips * nrows;
     if (grid > 0x7fffffffLL) return fail(PLIFE_ERR_ARGUMENT, "grid of %lld workgroups too large", grid);
     if ((rc = launch_force_any(h, a, (int)grid, lds))) return rc;
@@ -701,6 +713,26 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs)
     HIP_TRY(hipMemcpyAsync(&v, h->d_pairs, 8, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     *pairs = (int64_t)v;
+    return PLIFE_OK;
+}
+
+int plife_set_tuning(plife_physics *h, const plife_tuning *t)
+{
+    if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (t->strip_cells < 0 || t->strip_cells > 64 || t->list_cap < 0 || t->list_cap > 512 ||
+        !(t->stage_slack >= 0.0) || t->stage_slack > 16.0)
+        return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
+    h->tune = *t;
+    return PLIFE_OK;
+}
+
+int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
+{
+    if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    *t = h->tune;
+    if (t->strip_cells == 0) t->strip_cells = h->last_W;
+    if (t->list_cap == 0) t->list_cap = kListCap;
+    if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
     return PLIFE_OK;
 }
 

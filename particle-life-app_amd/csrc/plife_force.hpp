@@ -37,12 +37,13 @@ struct ForceArgs {
     const double *matrix;            // ntypes*ntypes, row = acted-on type
     int ntypes, m_in_lds;
     int n, nx, W, nstrips, cap, lcap, all_global;
-    float r2hi;                      // screen bound, units (2^-32)^2: (rmax 2^32)^2 (1 + 2^-20)
-    double r2;                       // rmax*rmax, the reference's comparison value
-    double rmax, inv_rmax;
+    float r2hi;                      // screen bound, units 2^-64: (rmax 2^32)^2 (1 + 2^-20)
+    double r2u;                      // fl(rmax*rmax) * 2^64: the reference's bound in units 2^-64
+    double S;                        // 2^-32 / rmax: |rel| = |D| S with D in units 2^-32
+    double R;                        // rmax * 2^32:  1/|rel| = R / |D|
     double beta, one_plus_beta, inv_1mb, inv_beta;
     double ff;                       // friction^(60 dt) (host FP64 pow, Physics.java:401)
-    double kdt;                      // force * dt
+    double kdt;                      // force * dt * 2^-32
     double cs, dt;
     unsigned long long *pair_counter;
     int32_t *dbg_count;
@@ -63,31 +64,30 @@ __device__ __forceinline__ unsigned long long id_hash(uint32_t id)
     return z ^ (z >> 31);
 }
 
-// One pair in FP64.  With D = x_q - x_p (exact), rel = D/rmax, dist = |rel|,
+// One pair in FP64, in units of 2^-32 (the fixed-point grid), where scaling
+// by powers of two is exact: with D = x_q - x_p, rel = D/rmax, dist = |rel|,
 // the reference adds rel*(f/dist)*(rmax*force*dt) = D*(f/dist)*(force*dt) where
-//   f/dist = 1/beta - 1/dist                           (dist <  beta)
+//   f/dist = 1/beta - 1/dist                             (dist <  beta)
 //          = a*(1 - |1 + beta - 2 dist|/(1 - beta))/dist (dist >= beta)
-// (app/Main.java:275-280).  Here acc accumulates D*(f/dist).
+// (app/Main.java:275-280).  acc accumulates D*(f/dist) in units of 2^-32.
 template <int DIM, bool DEBUG>
 __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, double m, uint32_t qid,
                                        const ForceArgs &a)
 {
-    double dx, dy, dz, d2;
+    const double dx = (double)dxi, dy = (double)dyi, dz = (double)dzi;
+    double d2;
     {
 #pragma clang fp contract(off)
-        dx = (double)dxi * kInvFix;
-        dy = (double)dyi * kInvFix;
-        dz = DIM == 3 ? (double)dzi * kInvFix : 0.0;
-        // JOML lengthSquared without FMA: x*x + (y*y + z*z)
+        // JOML lengthSquared without FMA: x*x + (y*y + z*z) (exact 2^64 scaling)
         d2 = DIM == 3 ? dx * dx + (dy * dy + dz * dz) : dx * dx + dy * dy;
     }
-    const bool in = (d2 != 0.0) & (d2 <= a.r2);
+    const bool in = (d2 != 0.0) & (d2 <= a.r2u);     // Physics.java:432
     const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
     const double e = d2 * y0;
     const double h = fma(-e, y0, 1.0);
-    const double y = fma(0.5 * y0, h, y0);      // 1/|D|, one Newton step from FP32
-    const double dist = (d2 * y) * a.inv_rmax;
-    const double inv = a.rmax * y;              // 1/dist
+    const double y = fma(0.5 * y0, h, y0);            // 1/|D|, one Newton step from FP32
+    const double dist = (d2 * y) * a.S;
+    const double inv = y * a.R;                        // 1/dist
     const double t = fabs(fma(dist, -2.0, a.one_plus_beta));
     const double g = fma(t, -a.inv_1mb, 1.0);
     const double att = (m * g) * inv;
@@ -101,6 +101,13 @@ __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, doub
     if constexpr (DEBUG) {
         if (in) acc.hash += id_hash(qid);
     }
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
+    return v;
 }
 
 // Phase A screen on exact fixed-point deltas.  In a clamped world the int32
@@ -221,11 +228,36 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
             }
 }
 
+// LDS layout of k_force, shared with the host's size computation.
+struct ForceLds {
+    size_t m, sxy, lists, sz, sid, off, gs, st, total;
+};
+
+__host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int ntypes, bool m_in_lds,
+                                                     int cap, int lcap, int W)
+{
+    const int NR = dim == 3 ? 9 : 3;
+    ForceLds L;
+    size_t o = 0;
+    L.m = o;     o += m_in_lds ? (size_t)ntypes * ntypes * 8 : 0;
+    L.sxy = o;   o += (size_t)cap * 8;
+    L.lists = o; o += (size_t)lcap * kBlock * 2;
+    o = (o + 3) & ~(size_t)3;
+    L.sz = o;    o += dim == 3 ? (size_t)cap * 4 : 0;
+    L.sid = o;   o += debug ? (size_t)cap * 4 : 0;
+    L.off = o;   o += (size_t)(NR * (W + 2) + 1) * 4;
+    L.gs = o;    o += (size_t)(NR * (W + 2)) * 4;
+    L.st = o;    o += (size_t)cap;
+    L.total = (o + 15) & ~(size_t)15;
+    return L;
+}
+
 template <int DIM, bool WRAP, bool DEBUG>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
     constexpr int NR = DIM == 3 ? 9 : 3;      // stencil rows
     constexpr int RC = DIM == 3 ? 4 : 1;      // centre row (oy = oz = 0)
+    constexpr int U = 4;                      // screen unroll; lists keep U slots free
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t s_w[4];
     __shared__ unsigned long long s_w64[4];
@@ -239,13 +271,15 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const int rc = wlen + 2;                  // staged cells per row
     const int NC = NR * rc;
 
-    // LDS: stage[cap] uint4 | ids3[cap] (3D debug) | lists[lcap][256] u16 | M[T*T] f64 | off[] | gs[]
-    uint4 *stage = reinterpret_cast<uint4 *>(smem);
-    uint32_t *ids3 = reinterpret_cast<uint32_t *>(stage + a.cap);
-    uint16_t *lists = reinterpret_cast<uint16_t *>(ids3 + ((DIM == 3 && DEBUG) ? a.cap : 0));
-    double *M = reinterpret_cast<double *>(lists + a.lcap * kBlock);
-    uint32_t *off = reinterpret_cast<uint32_t *>(M + (a.m_in_lds ? a.ntypes * a.ntypes : 0));
-    uint32_t *gs = off + (NR * (a.W + 2) + 1);
+    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes, a.m_in_lds != 0, a.cap, a.lcap, a.W);
+    double *M = reinterpret_cast<double *>(smem + L.m);
+    uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
+    uint16_t *lists = reinterpret_cast<uint16_t *>(smem + L.lists);
+    uint32_t *szz = reinterpret_cast<uint32_t *>(smem + L.sz);
+    uint32_t *sid = reinterpret_cast<uint32_t *>(smem + L.sid);
+    uint32_t *off = reinterpret_cast<uint32_t *>(smem + L.off);
+    uint32_t *gs = reinterpret_cast<uint32_t *>(smem + L.gs);
+    uint8_t *st = reinterpret_cast<uint8_t *>(smem + L.st);
 
     const int tid = threadIdx.x;
     if (a.m_in_lds)
@@ -298,39 +332,34 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
                 else hi = mid;
             }
             const int g = (int)(gs[lo] + ((uint32_t)s - off[lo]));
-            uint4 e;
-            e.x = a.sx[g];
-            e.y = a.sy[g];
-            if constexpr (DIM == 3) {
-                e.z = a.sz[g];
-                e.w = (uint32_t)a.type[g];
-                if constexpr (DEBUG) ids3[s] = (uint32_t)a.id[g];
-            } else {
-                e.z = (uint32_t)a.type[g];
-                e.w = DEBUG ? (uint32_t)a.id[g] : 0u;
-            }
-            stage[s] = e;
+            sxy[s] = make_uint2(a.sx[g], a.sy[g]);
+            if constexpr (DIM == 3) szz[s] = a.sz[g];
+            if constexpr (DEBUG) sid[s] = (uint32_t)a.id[g];
+            st[s] = (uint8_t)a.type[g];
         }
     }
     __syncthreads();
 
-    const uint2 *stage2 = reinterpret_cast<const uint2 *>(stage);
     long long my_pairs = 0;
-    for (int hh = tid; hh < nhome; hh += kBlock) {
-        const int g = ghome + hh;
+    // all lanes iterate together (inactive lanes carry empty ranges) so the
+    // wave-uniform candidate loops and list flushes below see every lane
+    for (int base = 0; base < nhome; base += kBlock) {
+        const int hh = base + tid;
+        const bool active = hh < nhome;
+        const int g = ghome + (active ? hh : 0);
         Acc acc;
         acc.x = acc.y = acc.z = 0.0;
         acc.cnt = 0;
         acc.hash = 0ull;
-        uint32_t hx, hy, hz;
-        int hrow;
+        uint32_t hx = 0, hy = 0, hz = 0;
+        int hrow = 0;
         if (use_lds) {
-            const int hl = (int)off[fh] + hh;
-            const uint4 e = stage[hl];
+            const int hl = (int)off[fh] + (active ? hh : 0);
+            const uint2 e = sxy[hl];
             hx = e.x;
             hy = e.y;
-            hz = DIM == 3 ? e.z : 0u;
-            hrow = (int)(DIM == 3 ? e.w : e.z) * a.ntypes;
+            if constexpr (DIM == 3) hz = szz[hl];
+            hrow = (int)st[hl] * a.ntypes;
             // home column c in [1, wlen]: off[RC*rc + c] <= hl < off[RC*rc + c + 1]
             int lo = 1, hi = wlen + 1;
             while (hi - lo > 1) {
@@ -352,51 +381,51 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
             auto flush = [&]() {
                 for (int j = 0; j < cnt; ++j) {
                     const int k = lists[j * kBlock + tid];
-                    const uint4 q = stage[k];
+                    const uint2 q = sxy[k];
                     const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
-                    const int dzi = DIM == 3 ? (int)(q.z - hz) : 0;
-                    const int qt = (int)(DIM == 3 ? q.w : q.z);
-                    const uint32_t qid = DEBUG ? (DIM == 3 ? ids3[k] : q.w) : 0u;
-                    const int idx = hrow + qt;
+                    const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
+                    const int idx = hrow + (int)st[k];
                     const double m = a.m_in_lds ? M[idx] : a.matrix[idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, qid, a);
+                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
                 }
                 cnt = 0;
             };
+#pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-                if ((q2y && oy == -1) || (q2z && oz == -1)) continue;
-                int k = (int)off[r * rc + c_lo];
-                const int khi = (int)off[r * rc + c + 2];
-                while (k < khi) {
-                    if (cnt > 0 && cnt + (khi - k) > a.lcap) flush();
-                    const int kend = min(khi, k + (a.lcap - cnt));
-                    for (; k < kend; ++k) {
-                        bool pass;
-                        if constexpr (DIM == 3) {
-                            const uint4 q = stage[k];
-                            pass = screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, q.z, a.r2hi);
-                        } else {
-                            const uint2 q = stage2[2 * k];
-                            pass = screen<DIM, WRAP>(hx, hy, 0u, q.x, q.y, 0u, a.r2hi);
-                        }
-                        if (pass) {
-                            lists[cnt * kBlock + tid] = (uint16_t)k;
-                            ++cnt;
+                const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
+                const int klo = (int)off[r * rc + c_lo];
+                const int len = skip ? 0 : (int)off[r * rc + c + 2] - klo;
+                const int K = wave_max(len);
+                for (int i = 0; i < K; i += U) {
+                    if (__any(cnt > a.lcap - U)) flush();
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int ii = i + u;
+                        if (ii < len) {
+                            const int k = klo + ii;
+                            const uint2 q = sxy[k];
+                            const uint32_t qz = DIM == 3 ? szz[k] : 0u;
+                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
+                                lists[cnt * kBlock + tid] = (uint16_t)k;
+                                ++cnt;
+                            }
                         }
                     }
                 }
             }
             flush();
-        } else {
+        } else if (active) {
             hx = a.sx[g];
             hy = a.sy[g];
             hz = DIM == 3 ? a.sz[g] : 0u;
             hrow = a.type[g] * a.ntypes;
             home_global<DIM, WRAP, DEBUG>(a, M, g, hx, hy, hz, hrow, acc);
         }
-        my_pairs += acc.cnt;
-        finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, g);
+        if (active) {
+            my_pairs += acc.cnt;
+            finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, g);
+        }
     }
 
     // interacting pairs, one 64-bit atomic per workgroup

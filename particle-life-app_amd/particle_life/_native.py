@@ -36,6 +36,8 @@ EXPORTS = (
     "plife_get_pair_count",
     "plife_set_debug",
     "plife_get_neighbor_summary",
+    "plife_set_tuning",
+    "plife_get_tuning",
     "plife_profile",
     "plife_time_steps",
 )
@@ -62,6 +64,14 @@ class PlifeTiming(ctypes.Structure):
         ("force_ms", ctypes.c_double),
         ("pairs", ctypes.c_int64),
         ("steps", ctypes.c_int32),
+    ]
+
+
+class PlifeTuning(ctypes.Structure):
+    _fields_ = [
+        ("strip_cells", ctypes.c_int32),
+        ("list_cap", ctypes.c_int32),
+        ("stage_slack", ctypes.c_double),
     ]
 
 
@@ -106,6 +116,8 @@ def load() -> ctypes.CDLL:
         "plife_get_pair_count": [vp, i64p],
         "plife_set_debug": [vp, i32],
         "plife_get_neighbor_summary": [vp, i32, vp, vp],
+        "plife_set_tuning": [vp, ctypes.POINTER(PlifeTuning)],
+        "plife_get_tuning": [vp, ctypes.POINTER(PlifeTuning), ctypes.POINTER(i32)],
         "plife_profile": [vp, i32, ctypes.POINTER(PlifeTiming)],
         "plife_time_steps": [vp, i32, ctypes.POINTER(ctypes.c_double)],
     }

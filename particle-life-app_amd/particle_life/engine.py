@@ -12,7 +12,7 @@ from typing import Optional
 import numpy as np
 
 from . import _native
-from ._native import PlifeSettings, PlifeTiming, check
+from ._native import PlifeSettings, PlifeTiming, PlifeTuning, check
 
 
 class Engine:
@@ -114,6 +114,18 @@ class Engine:
         h = np.zeros(self.n, np.uint64)
         check(self._lib.plife_get_neighbor_summary(self._h, self.n, count.ctypes.data, h.ctypes.data))
         return count, h
+
+    # launch tuning (results do not depend on it)
+    def set_tuning(self, strip_cells: int = 0, list_cap: int = 0, stage_slack: float = 0.0) -> None:
+        t = PlifeTuning(int(strip_cells), int(list_cap), float(stage_slack))
+        check(self._lib.plife_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = PlifeTuning()
+        lds = ctypes.c_int32()
+        check(self._lib.plife_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lds)))
+        return {"strip_cells": t.strip_cells, "list_cap": t.list_cap,
+                "stage_slack": t.stage_slack, "lds_bytes": lds.value}
 
     # timing
     def time_steps(self, nsteps: int) -> float:
