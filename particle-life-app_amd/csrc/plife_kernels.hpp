@@ -110,6 +110,23 @@ __device__ __forceinline__ int key_axis(uint32_t k, double cs, int nx)
     return c == nx ? nx - 1 : c;
 }
 
+// Cell counts are accumulated with one atomic per run of equal keys inside a
+// wave: the input is the previous step's cell order, so most lanes of a wave
+// share a few keys and per-lane atomics would serialise on the same address.
+__device__ __forceinline__ void count_runs(uint32_t key, bool valid, uint32_t *__restrict__ cell_count)
+{
+    const int lane = __lane_id();
+    const uint32_t prev = __shfl_up(key, 1, kWave);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = (int)__popcll(__ballot(valid));   // valid lanes are a prefix
+    if (head) {
+        const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const int next = later ? (int)__ffsll((long long)later) - 1 : nvalid;
+        atomicAdd(&cell_count[key], (uint32_t)(next - lane));
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_bin_keys(const uint32_t *__restrict__ px,
                                                      const uint32_t *__restrict__ py,
@@ -119,11 +136,14 @@ __global__ __launch_bounds__(kBlock) void k_bin_keys(const uint32_t *__restrict_
                                                      uint32_t *__restrict__ cell_count)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
-    if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
-    keys[i] = key;
-    atomicAdd(&cell_count[key], 1u);
+    const bool valid = i < n;
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
+        if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
+        keys[i] = key;
+    }
+    count_runs(key, valid, cell_count);
 }
 
 // ---------------------------------------------------------------------------
@@ -223,17 +243,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
     for (int t = threadIdx.x; t < R; t += kBlock) hist[t * ntiles + blockIdx.x] = h[t];
 }
 
+// ---------------------------------------------------------------------------
+// SoA state
+// ---------------------------------------------------------------------------
+struct PosBuf {
+    uint32_t *x, *y, *z;
+};
+struct AuxBuf {
+    float *vx, *vy, *vz;
+    int32_t *type, *id;
+};
+
 // Stable scatter.  Elements are ranked in index order: round r covers indices
 // base + r*256 + tid, and inside a round waves go in order and lanes go in
 // order (wave-level match by digit ballots).  gofs is the digit-major
 // exclusive scan of the per-tile histograms.  vin == nullptr means value = index.
-template <int D>
+//
+// GATHER (last pass): besides the permutation, copy the particle SoA from its
+// pre-sort slot to its sorted slot (the array swap of Physics.java:350-353).
+template <int D, int GATHER_DIM>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__restrict__ kin,
                                                           const uint32_t *__restrict__ vin,
                                                           uint32_t *__restrict__ kout,
                                                           uint32_t *__restrict__ vout,
                                                           const uint32_t *__restrict__ gofs,
-                                                          int n, int shift, int ntiles)
+                                                          int n, int shift, int ntiles,
+                                                          PosBuf ps, AuxBuf as, PosBuf pd, AuxBuf ad)
 {
     constexpr int R = 1 << D;
     constexpr int NW = kBlock / kWave;
@@ -309,21 +344,23 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
         const uint32_t k = s_k[j];
         const uint32_t d = (k >> shift) & (R - 1);
         const uint32_t g = s_gb[d] + (uint32_t)j - s_start[d];
+        const uint32_t v = s_v[j];
         if (kout) kout[g] = k;
-        vout[g] = s_v[j];
+        vout[g] = v;
+        if constexpr (GATHER_DIM != 0) {
+            pd.x[g] = ps.x[v];
+            pd.y[g] = ps.y[v];
+            ad.vx[g] = as.vx[v];
+            ad.vy[g] = as.vy[v];
+            if constexpr (GATHER_DIM == 3) {
+                pd.z[g] = ps.z[v];
+                ad.vz[g] = as.vz[v];
+            }
+            ad.type[g] = as.type[v];
+            ad.id[g] = as.id[v];
+        }
     }
 }
-
-// ---------------------------------------------------------------------------
-// SoA state
-// ---------------------------------------------------------------------------
-struct PosBuf {
-    uint32_t *x, *y, *z;
-};
-struct AuxBuf {
-    float *vx, *vy, *vz;
-    int32_t *type, *id;
-};
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_gather(int n, const uint32_t *__restrict__ perm,

@@ -72,6 +72,7 @@ class PlifeTuning(ctypes.Structure):
         ("strip_cells", ctypes.c_int32),
         ("list_cap", ctypes.c_int32),
         ("stage_slack", ctypes.c_double),
+        ("direct_launch", ctypes.c_int32),
     ]
 
 

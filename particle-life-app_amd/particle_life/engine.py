@@ -116,8 +116,9 @@ class Engine:
         return count, h
 
     # launch tuning (results do not depend on it)
-    def set_tuning(self, strip_cells: int = 0, list_cap: int = 0, stage_slack: float = 0.0) -> None:
-        t = PlifeTuning(int(strip_cells), int(list_cap), float(stage_slack))
+    def set_tuning(self, strip_cells: int = 0, list_cap: int = 0, stage_slack: float = 0.0,
+                   direct_launch: bool = False) -> None:
+        t = PlifeTuning(int(strip_cells), int(list_cap), float(stage_slack), 1 if direct_launch else 0)
         check(self._lib.plife_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -125,7 +126,8 @@ class Engine:
         lds = ctypes.c_int32()
         check(self._lib.plife_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lds)))
         return {"strip_cells": t.strip_cells, "list_cap": t.list_cap,
-                "stage_slack": t.stage_slack, "lds_bytes": lds.value}
+                "stage_slack": t.stage_slack, "direct_launch": bool(t.direct_launch),
+                "lds_bytes": lds.value}
 
     # timing
     def time_steps(self, nsteps: int) -> float:

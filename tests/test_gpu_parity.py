@@ -163,3 +163,26 @@ def test_full_size_c2_one_step(eng, which):
     st = check_one_step(eng, w, nthreads=16, debug=True)
     print(f"  {st}")
     assert 70 * w.n < st["pairs"] < 90 * w.n
+
+
+def test_graph_replay_matches_direct_launches(eng):
+    """Steps replayed from the captured HIP graph give bit-identical states to
+    direct launches, replay after replay, with eager work in between."""
+    from particle_life import Engine, worlds
+
+    print("\n[parity] case=graph_vs_direct", flush=True)
+    w = worlds.uniform_world(116, 20_000, 6, rmax=0.02, dt=0.01)
+    direct = Engine(0)
+    direct.set_tuning(direct_launch=True)
+    direct.load_world(w)
+    eng.set_tuning()
+    eng.load_world(w)
+    for rep in range(4):
+        eng.step(6)                      # three graph replays
+        for _ in range(6):
+            direct.step(1)
+        a, b = eng.state(), direct.state()
+        for k in ("pos", "vel", "ids"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"replay round {rep}: {k}")
+        assert eng.pair_count() == direct.pair_count()
+    direct.close()
