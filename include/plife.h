@@ -124,22 +124,22 @@ int plife_set_debug(plife_physics *h, int32_t on);
 int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash);
 
 /* Launch shape (no counterpart in the reference, which has no device).  Zero
- * fields mean automatic.  strip_cells in [0, 64]: home cells per force
- * workgroup strip; list_cap in [0, 512]: per-thread LDS candidate-list
- * entries; stage_slack in [0, 16]: LDS staging capacity as a multiple of the
- * expected staged particles; direct_launch 1: launch every kernel directly
- * instead of replaying a HIP graph of two steps.  Results do not depend on
- * the tuning. */
+ * fields mean automatic.  block_homes in {0, 64, 128, 256}: home particles
+ * (threads) per force workgroup; list_cap in [0, 512]: per-thread LDS
+ * candidate-list entries of the force kernel; stage_slack in [0, 16]: LDS
+ * staging capacity as a multiple of the expected staged particles;
+ * direct_launch 1: launch every kernel directly instead of replaying a HIP
+ * graph of two steps.  Results do not depend on the tuning. */
 typedef struct {
-    int32_t strip_cells;
+    int32_t block_homes;
     int32_t list_cap;
-    double stage_slack;
     int32_t direct_launch;
+    double stage_slack;
 } plife_tuning;
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t);
-/* Current tuning with automatic fields resolved from the last step, and the
- * LDS bytes per workgroup of the last force launch. */
+/* Current tuning with automatic fields resolved, and the LDS bytes per
+ * workgroup of the last force launch. */
 int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes);
 
 /* Runs nsteps steps with HIP events around each phase and fills *out. */

@@ -51,8 +51,7 @@ int fail(int code, const char *fmt, ...)
 
 constexpr int kMaxTypes = 256;        // the app caps the matrix size at 256 (app/Main.java:745)
 constexpr int kMaxTypesLds = 32;      // matrix staged in LDS up to 32x32 (8 KB of FP64)
-constexpr int kListCap = 24;          // per-thread candidate list entries (u16) in LDS
-constexpr double kHomesPerStrip = 200; // target home particles per 256-thread workgroup
+constexpr int kListCap = 32;          // per-thread candidate list entries (u16) in LDS
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxLds = 65536;
 
@@ -101,6 +100,7 @@ struct plife_physics {
 
     // binning / sort scratch
     uint32_t *keys0 = nullptr, *kA = nullptr, *kB = nullptr, *vA = nullptr, *vB = nullptr;
+    uint32_t *skeys = nullptr;         // cell key of each sorted slot (last radix pass)
     uint32_t *perm = nullptr;          // points at vA or vB after a step
     long long cells_cap = 0;
     uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_end = nullptr;
@@ -128,7 +128,7 @@ struct plife_physics {
     // launch tuning (0 = automatic) and what the last force launch used
     plife_tuning tune{};
     size_t last_lds = 0;
-    int last_W = 0, last_cap = 0;
+    int last_cap = 0;
 
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
@@ -159,6 +159,7 @@ void free_state(plife_physics *h)
     dev_free(h->kB);
     dev_free(h->vA);
     dev_free(h->vB);
+    dev_free(h->skeys);
     h->perm = nullptr;
     h->cap = 0;
 }
@@ -184,6 +185,7 @@ int ensure_capacity(plife_physics *h, int n)
     if ((rc = dev_alloc(&h->kB, cap))) return rc;
     if ((rc = dev_alloc(&h->vA, cap))) return rc;
     if ((rc = dev_alloc(&h->vB, cap))) return rc;
+    if ((rc = dev_alloc(&h->skeys, cap))) return rc;
     h->cap = cap;
     return PLIFE_OK;
 }
@@ -291,7 +293,7 @@ int radix_sort(plife_physics *h, int n, long long ncells, int dim, const PosBuf 
     const uint32_t *vin = nullptr;
     for (int p = 0; p < passes; ++p) {
         const bool last = p == passes - 1;
-        uint32_t *kout = last ? nullptr : ((p & 1) ? h->kB : h->kA);
+        uint32_t *kout = last ? h->skeys : ((p & 1) ? h->kB : h->kA);
         uint32_t *vout = (p & 1) ? h->vB : h->vA;
         const int shift = p * D;
         if (!last)
@@ -321,22 +323,34 @@ Grid grid_of(const plife_settings &s)
     return g;
 }
 
-template <int DIM, bool WRAP, bool DEBUG>
-void launch_force(plife_physics *h, const ForceArgs &a, int grid, size_t lds)
+template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
+void launch_force_bs(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
 {
-    k_force<DIM, WRAP, DEBUG><<<grid, kBlock, lds, h->stream>>>(a);
+    const long long grid = ((long long)a.n + bs - 1) / bs;
+    switch (bs) {
+        case 64: k_force<DIM, WRAP, DEBUG, MLDS, 64><<<(int)grid, 64, lds, h->stream>>>(a); break;
+        case 128: k_force<DIM, WRAP, DEBUG, MLDS, 128><<<(int)grid, 128, lds, h->stream>>>(a); break;
+        default: k_force<DIM, WRAP, DEBUG, MLDS, 256><<<(int)grid, 256, lds, h->stream>>>(a); break;
+    }
 }
 
-int launch_force_any(plife_physics *h, const ForceArgs &a, int grid, size_t lds)
+template <int DIM, bool WRAP, bool DEBUG>
+void launch_force(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
+{
+    if (a.m_in_lds) launch_force_bs<DIM, WRAP, DEBUG, true>(h, a, bs, lds);
+    else launch_force_bs<DIM, WRAP, DEBUG, false>(h, a, bs, lds);
+}
+
+int launch_force_any(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
 {
     const int dim = h->st.dim;
     const bool wrap = h->st.wrap != 0, dbg = h->debug != 0;
     if (dim == 2) {
-        if (wrap) dbg ? launch_force<2, true, true>(h, a, grid, lds) : launch_force<2, true, false>(h, a, grid, lds);
-        else dbg ? launch_force<2, false, true>(h, a, grid, lds) : launch_force<2, false, false>(h, a, grid, lds);
+        if (wrap) dbg ? launch_force<2, true, true>(h, a, bs, lds) : launch_force<2, true, false>(h, a, bs, lds);
+        else dbg ? launch_force<2, false, true>(h, a, bs, lds) : launch_force<2, false, false>(h, a, bs, lds);
     } else {
-        if (wrap) dbg ? launch_force<3, true, true>(h, a, grid, lds) : launch_force<3, true, false>(h, a, grid, lds);
-        else dbg ? launch_force<3, false, true>(h, a, grid, lds) : launch_force<3, false, false>(h, a, grid, lds);
+        if (wrap) dbg ? launch_force<3, true, true>(h, a, bs, lds) : launch_force<3, true, false>(h, a, bs, lds);
+        else dbg ? launch_force<3, false, true>(h, a, bs, lds) : launch_force<3, false, false>(h, a, bs, lds);
     }
     LAUNCH_CHECK();
     return PLIFE_OK;
@@ -419,24 +433,22 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.nx = gr.nx;
     const double avg = (double)n / (double)gr.ncells;
     const int NR = s.dim == 3 ? 9 : 3;
-    const int nrows = s.dim == 3 ? gr.nx * gr.nx : gr.nx;
-    // launch shape: strips of W cells holding about kHomesPerStrip homes
-    int W = h->tune.strip_cells > 0 ? h->tune.strip_cells
-                                    : (int)std::lround(kHomesPerStrip / std::max(avg, 1e-9));
-    W = std::max(1, std::min(W, 64));
-    W = std::min(W, gr.nx);
-    a.W = W;
-    a.nstrips = ceil_div(gr.nx, W);
     const bool m_in_lds = h->ntypes <= kMaxTypesLds;
     a.m_in_lds = m_in_lds ? 1 : 0;
+    a.skeys = h->skeys;
     a.lcap = h->tune.list_cap > 0 ? std::max(8, std::min(h->tune.list_cap, 512)) : kListCap;
+    // a workgroup of BS homes spans ~BS/avg + 1 cells of a row; it stages NR
+    // rows of (span + 2) cells
+    const int bs = h->tune.block_homes == 64 || h->tune.block_homes == 128 ? h->tune.block_homes : 256;
     const double slack = h->tune.stage_slack > 0 ? h->tune.stage_slack : 1.25;
-    long long want = (long long)std::ceil(NR * (W + 2) * avg * slack) + 128;
+    const double span = bs / std::max(avg, 1e-9) + 1.0;
+    a.max_cells = (int)std::min<double>(std::max(64.0, std::ceil(NR * (span + 2.0) * 1.5 + 16)), 2048.0);
+    long long want = (long long)std::ceil(NR * (span + 2.0) * avg * slack) + 64;
     want = std::min<long long>((want + 63) / 64 * 64, 65535);
     // shrink the staging capacity until the layout fits the LDS budget
     const bool dbg = h->debug != 0;
-    while (want > 0 && force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, (int)want, a.lcap, W).total >
-                           (size_t)kMaxLds)
+    while (want > 0 && force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, (int)want, a.lcap, bs,
+                                        a.max_cells).total > (size_t)kMaxLds)
         want -= 64;
     a.cap = (int)std::max(0LL, want);
     a.all_global = gr.nx < 3 ? 1 : 0;
@@ -455,13 +467,11 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.pair_counter = h->d_pairs;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
-    const size_t lds = force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, a.cap, a.lcap, W).total;
+    const size_t lds = force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, a.cap, a.lcap, bs,
+                                        a.max_cells).total;
     h->last_lds = lds;
-    h->last_W = W;
     h->last_cap = a.cap;
-    const long long grid = (long long)a.nstrips * nrows;
-    if (grid > 0x7fffffffLL) return fail(PLIFE_ERR_ARGUMENT, "grid of %lld workgroups too large", grid);
-    if ((rc = launch_force_any(h, a, (int)grid, lds))) return rc;
+    if ((rc = launch_force_any(h, a, bs, lds))) return rc;
     if ((rc = record(h, ev, 4))) return rc;
     h->ac = ao;
     return PLIFE_OK;
@@ -833,9 +843,10 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs)
 int plife_set_tuning(plife_physics *h, const plife_tuning *t)
 {
     if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
-    if (t->strip_cells < 0 || t->strip_cells > 64 || t->list_cap < 0 || t->list_cap > 512 ||
-        !(t->stage_slack >= 0.0) || t->stage_slack > 16.0 || t->direct_launch < 0 ||
-        t->direct_launch > 1)
+    if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
+        t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
+        (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
+         t->block_homes != 256))
         return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
     h->tune = *t;
     return PLIFE_OK;
@@ -845,8 +856,8 @@ int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
 {
     if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     *t = h->tune;
-    if (t->strip_cells == 0) t->strip_cells = h->last_W;
     if (t->list_cap == 0) t->list_cap = kListCap;
+    if (t->block_homes == 0) t->block_homes = 256;
     if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
     return PLIFE_OK;
 }

@@ -3,12 +3,13 @@
 // Reference: backend/Physics.java:397-450 (updateVelocity, updatePosition),
 // the accelerator lambda app/Main.java:275-280, Range.java:40-96.
 //
-// Work decomposition: one 256-thread workgroup per strip of W cells along x in
-// one grid row (2D: row = cy, 3D: row = cy + cz*nx).  The workgroup stages the
-// 3^(DIM-1) neighbour rows x (W+2) cells of the cell-sorted SoA in LDS; each
-// thread then owns one home particle of the strip and, per stencil row, scans
-// the three staged cells around its own column -- exactly the cells the
-// reference's 3x3 (3x3x3) stencil visits.
+// Work decomposition: a workgroup of BS threads owns BS consecutive home
+// particles of the cell-sorted order, so every wave is full.  It stages, for
+// each grid row it touches (2D: row = cy, 3D: row = cy + cz*nx), the
+// 3^(DIM-1) neighbour rows x (span + 2) cells of the sorted SoA in LDS; each
+// thread owns one home and, per stencil row, scans the three staged cells
+// around its own column -- exactly the cells the reference's 3x3 (3x3x3)
+// stencil visits.
 //
 // Each home particle runs two phases over its candidates:
 //   A  FP32 screen: exact int32 minimum-image deltas (positions are 32-bit
@@ -33,10 +34,11 @@ struct ForceArgs {
     const int32_t *type, *id;
     // new positions, same (sorted) order
     uint32_t *ox, *oy, *oz;
+    const uint32_t *skeys;           // cell key of each sorted slot
     const uint32_t *cell_start, *cell_end;
     const double *matrix;            // ntypes*ntypes, row = acted-on type
     int ntypes, m_in_lds;
-    int n, nx, W, nstrips, cap, lcap, all_global;
+    int n, nx, cap, lcap, max_cells, all_global;
     float r2hi;                      // screen bound, units 2^-64: (rmax 2^32)^2 (1 + 2^-20)
     double r2u;                      // fl(rmax*rmax) * 2^64: the reference's bound in units 2^-64
     double S;                        // 2^-32 / rmax: |rel| = |D| S with D in units 2^-32
@@ -187,7 +189,7 @@ __device__ __forceinline__ void finish_home(const ForceArgs &a, const Acc &acc, 
 // (Physics.java:404-424), FP64 throughout.  Used for strips whose staged
 // neighbourhood does not fit in LDS and for grids with fewer than 3 cells per
 // axis (where the reference visits repeated cells).
-template <int DIM, bool WRAP, bool DEBUG>
+template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
 __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint32_t hx,
                             uint32_t hy, uint32_t hz, int row, Acc &acc)
 {
@@ -222,56 +224,57 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
                         if (!ok) continue;
                     }
                     const int idx = row + a.type[j];
-                    const double m = a.m_in_lds ? Mlds[idx] : a.matrix[idx];
+                    const double m = MLDS ? Mlds[idx] : a.matrix[idx];
                     pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[j] : 0u, a);
                 }
             }
 }
 
-// LDS layout of k_force, shared with the host's size computation.
+// A workgroup of BS threads (BS/64 waves) owns BS consecutive home particles
+// of the cell-sorted order, so every wave is full.  The homes span a few
+// consecutive cells of one grid row, or (at row ends) of a few rows: each
+// distinct row is a "part" whose 3^(DIM-1) stencil rows x (span + 2) cells are
+// staged in LDS once for the whole workgroup.  Workgroups with more parts or
+// staged cells than the LDS budget take the global path.
+constexpr int kMaxParts = 4;
+
 struct ForceLds {
     size_t m, sxy, lists, sz, sid, off, gs, st, total;
 };
 
 __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int ntypes, bool m_in_lds,
-                                                     int cap, int lcap, int W)
+                                                     int cap, int lcap, int bs, int max_cells)
 {
-    const int NR = dim == 3 ? 9 : 3;
     ForceLds L;
     size_t o = 0;
     L.m = o;     o += m_in_lds ? (size_t)ntypes * ntypes * 8 : 0;
     L.sxy = o;   o += (size_t)cap * 8;
-    L.lists = o; o += (size_t)lcap * kBlock * 2;
+    L.lists = o; o += (size_t)lcap * bs * 2;
     o = (o + 3) & ~(size_t)3;
     L.sz = o;    o += dim == 3 ? (size_t)cap * 4 : 0;
     L.sid = o;   o += debug ? (size_t)cap * 4 : 0;
-    L.off = o;   o += (size_t)(NR * (W + 2) + 1) * 4;
-    L.gs = o;    o += (size_t)(NR * (W + 2)) * 4;
+    L.off = o;   o += (size_t)(max_cells + 1) * 4;
+    L.gs = o;    o += (size_t)max_cells * 4;
     L.st = o;    o += (size_t)cap;
     L.total = (o + 15) & ~(size_t)15;
     return L;
 }
 
-template <int DIM, bool WRAP, bool DEBUG>
-__global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
+// MLDS: the type matrix is staged in LDS (ntypes <= 32); otherwise it is read
+// from global memory.  A compile-time switch, so the hot loop's matrix load is
+// a plain ds_read and never a generic (flat) load.
+template <int DIM, bool WRAP, bool DEBUG, bool MLDS, int BS>
+__global__ __launch_bounds__(BS) void k_force(ForceArgs a)
 {
-    constexpr int NR = DIM == 3 ? 9 : 3;      // stencil rows
-    constexpr int RC = DIM == 3 ? 4 : 1;      // centre row (oy = oz = 0)
+    constexpr int NR = DIM == 3 ? 9 : 3;      // stencil rows per part
     constexpr int U = 4;                      // screen unroll; lists keep U slots free
+    constexpr int NW = BS / kWave;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ uint32_t s_w[4];
-    __shared__ unsigned long long s_w64[4];
+    __shared__ int s_part[kMaxParts][3];      // row, first column, last column
+    __shared__ uint32_t s_cnt[NW + 1];
+    __shared__ unsigned long long s_pairs[NW];
 
-    const int nx = a.nx;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int strip = b % a.nstrips, row = b / a.nstrips;
-    const int cy = row % nx, cz = DIM == 3 ? row / nx : 0;
-    const int cx0 = strip * a.W;
-    const int wlen = min(a.W, nx - cx0);
-    const int rc = wlen + 2;                  // staged cells per row
-    const int NC = NR * rc;
-
-    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes, a.m_in_lds != 0, a.cap, a.lcap, a.W);
+    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes, MLDS, a.cap, a.lcap, BS, a.max_cells);
     double *M = reinterpret_cast<double *>(smem + L.m);
     uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
     uint16_t *lists = reinterpret_cast<uint16_t *>(smem + L.lists);
@@ -281,156 +284,194 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     uint32_t *gs = reinterpret_cast<uint32_t *>(smem + L.gs);
     uint8_t *st = reinterpret_cast<uint8_t *>(smem + L.st);
 
-    const int tid = threadIdx.x;
-    if (a.m_in_lds)
-        for (int t = tid; t < a.ntypes * a.ntypes; t += kBlock) M[t] = a.matrix[t];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const int nx = a.nx;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int g0 = blk * BS;
+    const int nact = min(BS, a.n - g0);
+    const bool active = tid < nact;
+    const int g = g0 + (active ? tid : nact - 1);
 
-    // per staged cell: global start and count, then a flat exclusive prefix
-    uint32_t cnt3[3];
-    uint32_t csum = 0;
+    if constexpr (MLDS)
+        for (int t = tid; t < a.ntypes * a.ntypes; t += BS) M[t] = a.matrix[t];
+
+    // this thread's home: grid row (2D: cy; 3D: cy + cz*nx) and column
+    const uint32_t hkey = a.skeys[g];
+    const int hrow = (int)(hkey / (uint32_t)nx);
+    const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
+    const uint32_t hx = a.sx[g], hy = a.sy[g], hz = DIM == 3 ? a.sz[g] : 0u;
+    const int mrow = a.type[g] * a.ntypes;
+
+    // parts = distinct rows among the homes (rows are non-decreasing with tid)
+    const int prow = tid > 0 ? (int)(a.skeys[g - 1] / (uint32_t)nx) : -1;
+    const bool pstart = active && (tid == 0 || prow != hrow);
+    const int nrow_next = (tid + 1 < nact) ? (int)(a.skeys[g + 1] / (uint32_t)nx) : -1;
+    const bool pend = active && (tid + 1 == nact || nrow_next != hrow);
+    const uint64_t wstarts = __ballot(pstart);
+    if (lane == 0) s_cnt[wv] = (uint32_t)__popcll(wstarts);
+    __syncthreads();
+    uint32_t before = 0, nparts = 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int f = tid * 3 + k;
-        uint32_t c = 0;
-        if (f < NC) {
-            const int r = f / rc, col = f - r * rc;
-            const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-            const int x = wrap_cell(cx0 - 1 + col, nx);
-            const int y = wrap_cell(cy + oy, nx);
-            const int z = DIM == 3 ? wrap_cell(cz + oz, nx) : 0;
-            const int ci = x + y * nx + z * nx * nx;
-            const uint32_t s0 = a.cell_start[ci];
-            c = a.cell_end[ci] - s0;
-            gs[f] = s0;
-        }
-        cnt3[k] = c;
-        csum += c;
+    for (int w = 0; w < NW; ++w) {
+        before += w < wv ? s_cnt[w] : 0u;
+        nparts += s_cnt[w];
     }
-    uint32_t total;
-    uint32_t o = block_excl_scan(csum, s_w, total);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int f = tid * 3 + k;
-        if (f < NC) off[f] = o;
-        o += cnt3[k];
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int mypart = (int)(before + __popcll(wstarts & le)) - 1;
+    if (pstart && mypart < kMaxParts) {
+        s_part[mypart][0] = hrow;
+        s_part[mypart][1] = hcx;
     }
-    if (tid == 0) off[NC] = total;
+    if (pend && mypart < kMaxParts) s_part[mypart][2] = hcx;
     __syncthreads();
 
-    const int S = (int)total;
-    const int fh = RC * rc + 1;                // first home cell (flat index)
-    const int nhome = (int)(off[fh + wlen] - off[fh]);
-    const int ghome = (int)gs[fh];             // home cells are contiguous in global memory
-    const bool use_lds = !a.all_global && S <= a.cap;
+    int p_row[kMaxParts], p_ca[kMaxParts], p_w[kMaxParts], p_base[kMaxParts + 1];
+    p_base[0] = 0;
+#pragma unroll
+    for (int p = 0; p < kMaxParts; ++p) {
+        const bool have = p < (int)nparts;
+        p_row[p] = have ? s_part[p][0] : 0;
+        p_ca[p] = have ? s_part[p][1] : 0;
+        p_w[p] = have ? s_part[p][2] - p_ca[p] + 3 : 0;     // span + 2 staged columns
+        p_base[p + 1] = p_base[p] + NR * p_w[p];
+    }
+    const int NC = p_base[kMaxParts];
+    bool use_lds = !a.all_global && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
 
+    // staged cells: start and count, then an exclusive prefix in LDS
+    uint32_t carry = 0;
     if (use_lds) {
-        for (int s = tid; s < S; s += kBlock) {
+        for (int f0 = 0; f0 < NC; f0 += BS) {
+            const int f = f0 + tid;
+            uint32_t c = 0;
+            if (f < NC) {
+                int pb = p_base[0], pw = p_w[0], pr = p_row[0], pc = p_ca[0];
+#pragma unroll
+                for (int q = 1; q < kMaxParts; ++q)
+                    if (f >= p_base[q] && p_w[q] > 0) { pb = p_base[q]; pw = p_w[q]; pr = p_row[q]; pc = p_ca[q]; }
+                const int r = (f - pb) / pw, j = (f - pb) - r * pw;
+                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
+                const int cy = pr % nx, cz = DIM == 3 ? pr / nx : 0;
+                const int x = wrap_cell(pc - 1 + j, nx);
+                const int y = wrap_cell(cy + oy, nx);
+                const int z = DIM == 3 ? wrap_cell(cz + oz, nx) : 0;
+                const int ci = x + y * nx + z * nx * nx;
+                const uint32_t s0 = a.cell_start[ci];
+                c = a.cell_end[ci] - s0;
+                gs[f] = s0;
+            }
+            const uint32_t inc = wave_incl_scan(c);
+            if (lane == kWave - 1) s_cnt[wv] = inc;
+            __syncthreads();
+            uint32_t wbase = carry, tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                wbase += w < wv ? s_cnt[w] : 0u;
+                tot += s_cnt[w];
+            }
+            if (f < NC) off[f] = wbase + inc - c;
+            carry += tot;
+            __syncthreads();
+        }
+        if (tid == 0) off[NC] = carry;
+        use_lds = (int)carry <= a.cap;
+    }
+    __syncthreads();
+
+    const int S = (int)carry;
+    if (use_lds) {
+        for (int s = tid; s < S; s += BS) {
             int lo = 0, hi = NC;               // off[lo] <= s < off[hi]
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (off[mid] <= (uint32_t)s) lo = mid;
                 else hi = mid;
             }
-            const int g = (int)(gs[lo] + ((uint32_t)s - off[lo]));
-            sxy[s] = make_uint2(a.sx[g], a.sy[g]);
-            if constexpr (DIM == 3) szz[s] = a.sz[g];
-            if constexpr (DEBUG) sid[s] = (uint32_t)a.id[g];
-            st[s] = (uint8_t)a.type[g];
+            const int q = (int)(gs[lo] + ((uint32_t)s - off[lo]));
+            sxy[s] = make_uint2(a.sx[q], a.sy[q]);
+            if constexpr (DIM == 3) szz[s] = a.sz[q];
+            if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
+            st[s] = (uint8_t)a.type[q];
         }
     }
     __syncthreads();
 
-    long long my_pairs = 0;
-    // all lanes iterate together (inactive lanes carry empty ranges) so the
-    // wave-uniform candidate loops and list flushes below see every lane
-    for (int base = 0; base < nhome; base += kBlock) {
-        const int hh = base + tid;
-        const bool active = hh < nhome;
-        const int g = ghome + (active ? hh : 0);
-        Acc acc;
-        acc.x = acc.y = acc.z = 0.0;
-        acc.cnt = 0;
-        acc.hash = 0ull;
-        uint32_t hx = 0, hy = 0, hz = 0;
-        int hrow = 0;
-        if (use_lds) {
-            const int hl = (int)off[fh] + (active ? hh : 0);
-            const uint2 e = sxy[hl];
-            hx = e.x;
-            hy = e.y;
-            if constexpr (DIM == 3) hz = szz[hl];
-            hrow = (int)st[hl] * a.ntypes;
-            // home column c in [1, wlen]: off[RC*rc + c] <= hl < off[RC*rc + c + 1]
-            int lo = 1, hi = wlen + 1;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (off[RC * rc + mid] <= (uint32_t)hl) lo = mid;
-                else hi = mid;
-            }
-            const int c = lo;
-            // Reference quirk: the stencil origin floor(x/cs) may be nx while the
-            // key is clamped to nx-1 (Physics.java:367-372 vs 404-405).  The
-            // reference then scans columns {nx-1, 0, 1}: skip staged column nx-2
-            // (column 1 is always farther than rmax).  Same for rows / layers.
-            const bool q2x = (cx0 + c - 1 == nx - 1) && stencil_origin(hx, a.cs) == nx;
-            const bool q2y = (cy == nx - 1) && stencil_origin(hy, a.cs) == nx;
-            const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, a.cs) == nx;
-            const int c_lo = c - 1 + (q2x ? 1 : 0);
-
-            int cnt = 0;
-            auto flush = [&]() {
-                for (int j = 0; j < cnt; ++j) {
-                    const int k = lists[j * kBlock + tid];
-                    const uint2 q = sxy[k];
-                    const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
-                    const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
-                    const int idx = hrow + (int)st[k];
-                    const double m = a.m_in_lds ? M[idx] : a.matrix[idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
-                }
-                cnt = 0;
-            };
-#pragma unroll 1
-            for (int r = 0; r < NR; ++r) {
-                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-                const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
-                const int klo = (int)off[r * rc + c_lo];
-                const int len = skip ? 0 : (int)off[r * rc + c + 2] - klo;
-                const int K = wave_max(len);
-                for (int i = 0; i < K; i += U) {
-                    if (__any(cnt > a.lcap - U)) flush();
+    Acc acc;
+    acc.x = acc.y = acc.z = 0.0;
+    acc.cnt = 0;
+    acc.hash = 0ull;
+    if (use_lds) {
+        int pb = p_base[0], pw = p_w[0], pc = p_ca[0];
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int ii = i + u;
-                        if (ii < len) {
-                            const int k = klo + ii;
-                            const uint2 q = sxy[k];
-                            const uint32_t qz = DIM == 3 ? szz[k] : 0u;
-                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
-                                lists[cnt * kBlock + tid] = (uint16_t)k;
-                                ++cnt;
-                            }
+        for (int q = 1; q < kMaxParts; ++q)
+            if (mypart == q) { pb = p_base[q]; pw = p_w[q]; pc = p_ca[q]; }
+        const int jh = hcx - pc + 1;           // staged column of the home cell
+        // Reference quirk: the stencil origin floor(x/cs) may be nx while the
+        // key is clamped to nx-1 (Physics.java:367-372 vs 404-405).  The
+        // reference then scans columns {nx-1, 0, 1}: skip staged column nx-2
+        // (column 1 is always farther than rmax).  Same for rows / layers.
+        const int cy = hrow % nx, cz = DIM == 3 ? hrow / nx : 0;
+        const bool q2x = (hcx == nx - 1) && stencil_origin(hx, a.cs) == nx;
+        const bool q2y = (cy == nx - 1) && stencil_origin(hy, a.cs) == nx;
+        const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, a.cs) == nx;
+        const int j_lo = jh - 1 + (q2x ? 1 : 0);
+
+        int cnt = 0;
+        auto flush = [&]() {
+            for (int j = 0; j < cnt; ++j) {
+                const int k = lists[j * BS + tid];
+                const uint2 q = sxy[k];
+                const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
+                const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
+                const int idx = mrow + (int)st[k];
+                const double m = MLDS ? M[idx] : a.matrix[idx];
+                pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
+            }
+            cnt = 0;
+        };
+#pragma unroll 1
+        for (int r = 0; r < NR; ++r) {
+            const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
+            const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
+            const int rb = pb + r * pw;
+            const int klo = (int)off[rb + j_lo];
+            const int len = skip ? 0 : (int)off[rb + jh + 2] - klo;
+            const int K = wave_max(len);
+            for (int i = 0; i < K; i += U) {
+                if (__any(cnt > a.lcap - U)) flush();
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int ii = i + u;
+                    if (ii < len) {
+                        const int k = klo + ii;
+                        const uint2 q = sxy[k];
+                        const uint32_t qz = DIM == 3 ? szz[k] : 0u;
+                        if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
+                            lists[cnt * BS + tid] = (uint16_t)k;
+                            ++cnt;
                         }
                     }
                 }
             }
-            flush();
-        } else if (active) {
-            hx = a.sx[g];
-            hy = a.sy[g];
-            hz = DIM == 3 ? a.sz[g] : 0u;
-            hrow = a.type[g] * a.ntypes;
-            home_global<DIM, WRAP, DEBUG>(a, M, g, hx, hy, hz, hrow, acc);
         }
-        if (active) {
-            my_pairs += acc.cnt;
-            finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, g);
-        }
+        flush();
+    } else if (active) {
+        home_global<DIM, WRAP, DEBUG, MLDS>(a, M, g, hx, hy, hz, mrow, acc);
     }
+    if (active) finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, g);
 
     // interacting pairs, one 64-bit atomic per workgroup
-    const unsigned long long tot_pairs = block_sum_u64((unsigned long long)my_pairs, s_w64);
-    if (tid == 0 && tot_pairs) atomicAdd(a.pair_counter, tot_pairs);
+    unsigned long long tot = active ? (unsigned long long)acc.cnt : 0ull;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) tot += __shfl_xor(tot, d, kWave);
+    if (lane == 0) s_pairs[wv] = tot;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += s_pairs[w];
+        if (sum) atomicAdd(a.pair_counter, sum);
+    }
 }
 
 }  // namespace plife

@@ -69,10 +69,10 @@ class PlifeTiming(ctypes.Structure):
 
 class PlifeTuning(ctypes.Structure):
     _fields_ = [
-        ("strip_cells", ctypes.c_int32),
+        ("block_homes", ctypes.c_int32),
         ("list_cap", ctypes.c_int32),
-        ("stage_slack", ctypes.c_double),
         ("direct_launch", ctypes.c_int32),
+        ("stage_slack", ctypes.c_double),
     ]
 
 

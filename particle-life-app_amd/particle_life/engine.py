@@ -116,18 +116,18 @@ class Engine:
         return count, h
 
     # launch tuning (results do not depend on it)
-    def set_tuning(self, strip_cells: int = 0, list_cap: int = 0, stage_slack: float = 0.0,
-                   direct_launch: bool = False) -> None:
-        t = PlifeTuning(int(strip_cells), int(list_cap), float(stage_slack), 1 if direct_launch else 0)
+    def set_tuning(self, list_cap: int = 0, stage_slack: float = 0.0,
+                   direct_launch: bool = False, block_homes: int = 0) -> None:
+        t = PlifeTuning(int(block_homes), int(list_cap), 1 if direct_launch else 0,
+                        float(stage_slack))
         check(self._lib.plife_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = PlifeTuning()
         lds = ctypes.c_int32()
         check(self._lib.plife_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lds)))
-        return {"strip_cells": t.strip_cells, "list_cap": t.list_cap,
-                "stage_slack": t.stage_slack, "direct_launch": bool(t.direct_launch),
-                "lds_bytes": lds.value}
+        return {"block_homes": t.block_homes, "list_cap": t.list_cap, "stage_slack": t.stage_slack,
+                "direct_launch": bool(t.direct_launch), "lds_bytes": lds.value}
 
     # timing
     def time_steps(self, nsteps: int) -> float:
