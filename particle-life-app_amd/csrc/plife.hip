@@ -120,6 +120,7 @@ struct plife_physics {
     unsigned long long *dbg_hash = nullptr;
 
     std::atomic<int> stop{0};
+    bool cells_dirty = false;      // cell_count may hold stale counts
 
     // HIP graph of two consecutive steps (buffer roles repeat every 2 steps)
     hipGraphExec_t gexec = nullptr;
@@ -201,6 +202,9 @@ int ensure_cells(plife_physics *h, long long ncells)
     if ((rc = dev_alloc(&h->cell_count, (size_t)ncells))) return rc;
     if ((rc = dev_alloc(&h->cell_start, (size_t)ncells))) return rc;
     if ((rc = dev_alloc(&h->cell_end, (size_t)ncells))) return rc;
+    // the step keeps cell_count all-zero between steps (the scan clears it)
+    HIP_TRY(hipMemsetAsync(h->cell_count, 0, (size_t)ncells * 4, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     h->cells_cap = ncells;
     return PLIFE_OK;
 }
@@ -244,70 +248,111 @@ int scan_u32(plife_physics *h, const uint32_t *in, int n, uint32_t *excl, uint32
     return PLIFE_OK;
 }
 
-template <int D, int G>
-int radix_pass(plife_physics *h, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-               uint32_t *vout, int n, int shift, int ntiles, const PosBuf &ps, const AuxBuf &as,
-               const PosBuf &pd, const AuxBuf &ad)
+struct RadixPlan {
+    int bits, passes, D, ntiles;
+    size_t hist_len;
+};
+
+RadixPlan radix_plan(long long ncells, int n)
 {
-    k_radix_hist<D><<<ntiles, kBlock, 0, h->stream>>>(kin, n, shift, ntiles, h->hist);
-    int rc = scan_u32(h, h->hist, (1 << D) * ntiles, h->hist, nullptr);
+    RadixPlan p;
+    p.bits = 0;
+    while ((1LL << p.bits) < ncells) ++p.bits;
+    if (p.bits == 0) p.bits = 1;
+    p.passes = (p.bits + 7) / 8;
+    p.D = (p.bits + p.passes - 1) / p.passes;
+    p.ntiles = ceil_div(std::max(n, 1), kRadixTile);
+    p.hist_len = (size_t)(1 << p.D) * (size_t)p.ntiles;
+    return p;
+}
+
+ScanJob no_job() { return ScanJob{nullptr, 0, nullptr, nullptr, 0, nullptr}; }
+
+// One or two device-wide scans sharing three launches.
+int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
+{
+    if (a.n <= 0) std::swap(a, b);
+    if (a.n <= 0) return PLIFE_OK;
+    const int nb0 = ceil_div(a.n, kScanTile), nb1 = b.n > 0 ? ceil_div(b.n, kScanTile) : 0;
+    int rc = ensure_u32(&h->bsum, &h->bsum_cap, (size_t)(nb0 + nb1));
     if (rc) return rc;
-    k_radix_scatter<D, G><<<ntiles, kBlock, 0, h->stream>>>(kin, vin, kout, vout, h->hist, n, shift,
-                                                            ntiles, ps, as, pd, ad);
+    a.bsum = h->bsum;
+    b.bsum = h->bsum + nb0;
+    k_scan2_reduce<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
+    k_scan2_single<<<nb1 ? 2 : 1, kBlock, 0, h->stream>>>(a, b);
+    k_scan2_down<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
     LAUNCH_CHECK();
     return PLIFE_OK;
 }
 
-template <int G>
-int radix_pass_d(plife_physics *h, int D, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                 uint32_t *vout, int n, int shift, int ntiles, const PosBuf &ps, const AuxBuf &as,
-                 const PosBuf &pd, const AuxBuf &ad)
+template <int D>
+int radix_pass(plife_physics *h, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+               uint32_t *vout, int n, int shift, int ntiles, bool hist_ready)
 {
-    switch (D) {
-        case 1: return radix_pass<1, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 2: return radix_pass<2, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 3: return radix_pass<3, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 4: return radix_pass<4, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 5: return radix_pass<5, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 6: return radix_pass<6, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        case 7: return radix_pass<7, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        default: return radix_pass<8, G>(h, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+    int rc;
+    if (!hist_ready) {
+        k_radix_hist<D><<<ntiles, kBlock, 0, h->stream>>>(kin, n, shift, ntiles, h->hist);
+        if ((rc = scan_jobs(h, ScanJob{h->hist, (1 << D) * ntiles, h->hist, nullptr, 0, nullptr},
+                            no_job())))
+            return rc;
     }
+    k_radix_scatter<D><<<ntiles, kScatterBlock, 0, h->stream>>>(kin, vin, kout, vout, h->hist, n,
+                                                                 shift, ntiles);
+    LAUNCH_CHECK();
+    return PLIFE_OK;
 }
 
 // Stable sort of (keys0[i], i) by key -- the reference's counting-sort
-// permutation -- with the SoA gather (src -> dst) fused into the last pass.
-// Leaves the permutation in h->perm.
-int radix_sort(plife_physics *h, int n, long long ncells, int dim, const PosBuf &ps,
-               const AuxBuf &as, const PosBuf &pd, const AuxBuf &ad)
+// permutation (Physics.java:342-353).  The first pass's tile histogram is
+// already scanned in h->hist (k_bin_tile).  Leaves the permutation (sorted
+// slot -> pre-sort index) in h->perm and the sorted keys in h->skeys.
+int radix_sort(plife_physics *h, int n, const RadixPlan &plan)
 {
-    int bits = 0;
-    while ((1LL << bits) < ncells) ++bits;
-    if (bits == 0) bits = 1;
-    const int passes = (bits + 7) / 8;
-    const int D = (bits + passes - 1) / passes;
-    const int ntiles = ceil_div(n, kRadixTile);
-    int rc = ensure_u32(&h->hist, &h->hist_cap, (size_t)(1 << D) * (size_t)ntiles);
-    if (rc) return rc;
     const uint32_t *kin = h->keys0;
     const uint32_t *vin = nullptr;
-    for (int p = 0; p < passes; ++p) {
-        const bool last = p == passes - 1;
+    int rc;
+    for (int p = 0; p < plan.passes; ++p) {
+        const bool last = p == plan.passes - 1;
         uint32_t *kout = last ? h->skeys : ((p & 1) ? h->kB : h->kA);
         uint32_t *vout = (p & 1) ? h->vB : h->vA;
-        const int shift = p * D;
-        if (!last)
-            rc = radix_pass_d<0>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        else if (dim == 3)
-            rc = radix_pass_d<3>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
-        else
-            rc = radix_pass_d<2>(h, D, kin, vin, kout, vout, n, shift, ntiles, ps, as, pd, ad);
+        const int shift = p * plan.D;
+        const bool hr = p == 0;
+        switch (plan.D) {
+            case 1: rc = radix_pass<1>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 2: rc = radix_pass<2>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 3: rc = radix_pass<3>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 4: rc = radix_pass<4>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 5: rc = radix_pass<5>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 6: rc = radix_pass<6>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            case 7: rc = radix_pass<7>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+            default: rc = radix_pass<8>(h, kin, vin, kout, vout, n, shift, plan.ntiles, hr); break;
+        }
         if (rc) return rc;
         kin = kout;
         vin = vout;
     }
     h->perm = const_cast<uint32_t *>(vin);
     return PLIFE_OK;
+}
+
+template <int DIM>
+void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, double cs, int nx, int ntiles)
+{
+#define PLIFE_BIN(DD)                                                                         \
+    k_bin_tile<DIM, DD><<<ntiles, 1024, 0, h->stream>>>(P.x, P.y, P.z, n, cs, nx, ntiles,     \
+                                                          h->keys0, h->cell_count, h->hist,   \
+                                                          h->d_pairs)
+    switch (D) {
+        case 1: PLIFE_BIN(1); break;
+        case 2: PLIFE_BIN(2); break;
+        case 3: PLIFE_BIN(3); break;
+        case 4: PLIFE_BIN(4); break;
+        case 5: PLIFE_BIN(5); break;
+        case 6: PLIFE_BIN(6); break;
+        case 7: PLIFE_BIN(7); break;
+        default: PLIFE_BIN(8); break;
+    }
+#undef PLIFE_BIN
 }
 
 struct Grid {
@@ -378,18 +423,30 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     const int nc = (int)gr.ncells;
 
     if ((rc = record(h, ev, 0))) return rc;
-    // 1. bin: cell keys + histogram, scan into start/end offsets (Physics.java:319-340)
-    k_fill_u32<<<ceil_div(nc, kBlock), kBlock, 0, h->stream>>>(h->cell_count, 0u, nc);
-    k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 1);
+    const RadixPlan plan = radix_plan(gr.ncells, n);
+    if ((rc = ensure_u32(&h->hist, &h->hist_cap, plan.hist_len))) return rc;
+    // A step that failed between binning and scanning may leave counts behind.
+    if (h->cells_dirty) {
+        k_fill_u32<<<ceil_div(nc, kBlock), kBlock, 0, h->stream>>>(h->cell_count, 0u, nc);
+        LAUNCH_CHECK();
+    }
+    h->cells_dirty = true;
+    // 1. bin (Physics.java:319-340): cell keys, cell histogram, first radix
+    //    digit histogram; then the cell scan (start / end offsets, counters
+    //    cleared for the next step) and the digit scan in one launch
     const PosBuf P0 = h->P(h->pc);
     if (n > 0) {
-        if (s.dim == 3)
-            k_bin_keys<3><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(P0.x, P0.y, P0.z, n, s.rmax, gr.nx, h->keys0, h->cell_count);
-        else
-            k_bin_keys<2><<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(P0.x, P0.y, P0.z, n, s.rmax, gr.nx, h->keys0, h->cell_count);
+        if (s.dim == 3) launch_bin<3>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
+        else launch_bin<2>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
+    } else {
+        k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 1);
     }
     LAUNCH_CHECK();
-    if ((rc = scan_u32(h, h->cell_count, nc, h->cell_start, h->cell_end))) return rc;
+    if ((rc = scan_jobs(h, ScanJob{h->cell_count, nc, h->cell_start, h->cell_end, 1, nullptr},
+                        n > 0 ? ScanJob{h->hist, (int)plan.hist_len, h->hist, nullptr, 0, nullptr}
+                              : no_job())))
+        return rc;
+    h->cells_dirty = false;
     if ((rc = record(h, ev, 1))) return rc;
 
     h->stepped = true;
@@ -402,29 +459,35 @@ int do_step(plife_physics *h, PhaseEvents *ev)
         return PLIFE_OK;
     }
 
-    // 2+3. stable sort of (key, index) = the reference's counting-sort
-    // permutation, gathering the SoA into cell order in its last pass
-    const int po = 1 - h->pc, ao = 1 - h->ac;
-    if ((rc = radix_sort(h, n, gr.ncells, s.dim, h->P(h->pc), h->A(h->ac), h->P(po), h->A(ao))))
-        return rc;
+    // 2. stable sort of (key, index) = the reference's counting-sort permutation
+    if ((rc = radix_sort(h, n, plan))) return rc;
     if ((rc = record(h, ev, 2))) return rc;
     if ((rc = record(h, ev, 3))) return rc;
 
-    // 4. force + integrate (Physics.java:122-131, fused: Jacobi)
+    // (Physics.java:122-131 fused: the update is Jacobi)
     const double R = s.rmax * kFix;
     const double R2 = R * R;
+    // 3. force + integrate: reads the pre-sort state through the permutation
+    //    and writes the new state in cell order (the array swap, Physics.java:350-353)
+    const int pi = h->pc, ai = h->ac, po = 1 - h->pc, ao = 1 - h->ac;
     ForceArgs a{};
-    a.sx = h->px[po];
-    a.sy = h->py[po];
-    a.sz = h->pz[po];
-    a.vx = h->vx[ao];
-    a.vy = h->vy[ao];
-    a.vz = h->vz[ao];
-    a.type = h->type[ao];
-    a.id = h->id[ao];
-    a.ox = h->px[h->pc];
-    a.oy = h->py[h->pc];
-    a.oz = h->pz[h->pc];
+    a.perm = h->perm;
+    a.sx = h->px[pi];
+    a.sy = h->py[pi];
+    a.sz = h->pz[pi];
+    a.vx = h->vx[ai];
+    a.vy = h->vy[ai];
+    a.vz = h->vz[ai];
+    a.type = h->type[ai];
+    a.id = h->id[ai];
+    a.ox = h->px[po];
+    a.oy = h->py[po];
+    a.oz = h->pz[po];
+    a.ovx = h->vx[ao];
+    a.ovy = h->vy[ao];
+    a.ovz = h->vz[ao];
+    a.otype = h->type[ao];
+    a.oid = h->id[ao];
     a.cell_start = h->cell_start;
     a.cell_end = h->cell_end;
     a.matrix = h->d_matrix;
@@ -473,6 +536,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     h->last_cap = a.cap;
     if ((rc = launch_force_any(h, a, bs, lds))) return rc;
     if ((rc = record(h, ev, 4))) return rc;
+    h->pc = po;
     h->ac = ao;
     return PLIFE_OK;
 }
@@ -485,16 +549,10 @@ int prepare_step(plife_physics *h)
     int rc;
     if ((rc = ensure_cells(h, gr.ncells))) return rc;
     if ((rc = ensure_debug(h))) return rc;
-    if (h->n == 0) return PLIFE_OK;
-    int bits = 0;
-    while ((1LL << bits) < gr.ncells) ++bits;
-    if (bits == 0) bits = 1;
-    const int passes = (bits + 7) / 8;
-    const int D = (bits + passes - 1) / passes;
-    const int ntiles = ceil_div(h->n, kRadixTile);
-    const size_t hist = (size_t)(1 << D) * (size_t)ntiles;
-    if ((rc = ensure_u32(&h->hist, &h->hist_cap, hist))) return rc;
-    const size_t nb = (size_t)std::max(ceil_div(gr.ncells, kScanTile), ceil_div((long long)hist, kScanTile));
+    const RadixPlan plan = radix_plan(gr.ncells, h->n);
+    if ((rc = ensure_u32(&h->hist, &h->hist_cap, plan.hist_len))) return rc;
+    const size_t nb = (size_t)ceil_div(gr.ncells, kScanTile) +
+                      (size_t)ceil_div((long long)plan.hist_len, kScanTile);
     return ensure_u32(&h->bsum, &h->bsum_cap, nb);
 }
 

@@ -28,12 +28,15 @@
 namespace plife {
 
 struct ForceArgs {
-    // cell-sorted step-start state
+    // step-start state in the pre-sort order, read through the permutation
+    const uint32_t *perm;            // sorted slot -> pre-sort index
     const uint32_t *sx, *sy, *sz;
-    float *vx, *vy, *vz;             // updated in place (each particle writes its own)
+    const float *vx, *vy, *vz;
     const int32_t *type, *id;
-    // new positions, same (sorted) order
+    // the new state, written in the sorted order (Physics.particles after the swap)
     uint32_t *ox, *oy, *oz;
+    float *ovx, *ovy, *ovz;
+    int32_t *otype, *oid;
     const uint32_t *skeys;           // cell key of each sorted slot
     const uint32_t *cell_start, *cell_end;
     const double *matrix;            // ntypes*ntypes, row = acted-on type
@@ -84,24 +87,23 @@ __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, doub
         d2 = DIM == 3 ? dx * dx + (dy * dy + dz * dz) : dx * dx + dy * dy;
     }
     const bool in = (d2 != 0.0) & (d2 <= a.r2u);     // Physics.java:432
-    const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
-    const double e = d2 * y0;
-    const double h = fma(-e, y0, 1.0);
-    const double y = fma(0.5 * y0, h, y0);            // 1/|D|, one Newton step from FP32
-    const double dist = (d2 * y) * a.S;
-    const double inv = y * a.R;                        // 1/dist
-    const double t = fabs(fma(dist, -2.0, a.one_plus_beta));
-    const double g = fma(t, -a.inv_1mb, 1.0);
-    const double att = (m * g) * inv;
-    const double rep = a.inv_beta - inv;
-    double c = dist < a.beta ? rep : att;
-    c = in ? c : 0.0;
-    acc.x = fma(c, dx, acc.x);
-    acc.y = fma(c, dy, acc.y);
-    if constexpr (DIM == 3) acc.z = fma(c, dz, acc.z);
-    acc.cnt += in ? 1 : 0;
-    if constexpr (DEBUG) {
-        if (in) acc.hash += id_hash(qid);
+    if (in) {
+        const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
+        const double e = d2 * y0;
+        const double h = fma(-e, y0, 1.0);
+        const double y = fma(0.5 * y0, h, y0);        // 1/|D|, one Newton step from FP32
+        const double dist = (d2 * y) * a.S;
+        const double inv = y * a.R;                    // 1/dist
+        const double t = fabs(fma(dist, -2.0, a.one_plus_beta));
+        const double g = fma(t, -a.inv_1mb, 1.0);
+        const double att = (m * g) * inv;
+        const double rep = a.inv_beta - inv;
+        const double c = dist < a.beta ? rep : att;
+        acc.x = fma(c, dx, acc.x);
+        acc.y = fma(c, dy, acc.y);
+        if constexpr (DIM == 3) acc.z = fma(c, dz, acc.z);
+        acc.cnt += 1;
+        if constexpr (DEBUG) acc.hash += id_hash(qid);
     }
 }
 
@@ -129,7 +131,9 @@ __device__ __forceinline__ bool screen(uint32_t hx, uint32_t hy, uint32_t hz, ui
         const float dz = (float)dzi;
         d2 = fmaf(dz, dz, d2);
     }
-    bool pass = (d2 > 0.0f) & (d2 <= r2hi);
+    // d2 == 0 (the home itself, coincident particles) passes the screen; the
+    // FP64 predicate of phase B drops it
+    bool pass = d2 <= r2hi;
     if constexpr (!WRAP) {
         pass &= ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
         if constexpr (DIM == 3) pass &= ((dzi >= 0) == (qz >= hz));
@@ -166,19 +170,22 @@ __device__ __forceinline__ uint32_t integrate_axis(uint32_t k, double v, double 
 
 template <int DIM, bool WRAP, bool DEBUG>
 __device__ __forceinline__ void finish_home(const ForceArgs &a, const Acc &acc, uint32_t hx,
-                                            uint32_t hy, uint32_t hz, int g)
+                                            uint32_t hy, uint32_t hz, int htype, uint32_t o,
+                                            int g)
 {
-    const double nvx = a.ff * (double)a.vx[g] + a.kdt * acc.x;
-    const double nvy = a.ff * (double)a.vy[g] + a.kdt * acc.y;
-    a.vx[g] = (float)nvx;
-    a.vy[g] = (float)nvy;
+    const double nvx = a.ff * (double)a.vx[o] + a.kdt * acc.x;
+    const double nvy = a.ff * (double)a.vy[o] + a.kdt * acc.y;
+    a.ovx[g] = (float)nvx;
+    a.ovy[g] = (float)nvy;
     a.ox[g] = integrate_axis(hx, nvx, a.dt, WRAP);
     a.oy[g] = integrate_axis(hy, nvy, a.dt, WRAP);
     if constexpr (DIM == 3) {
-        const double nvz = a.ff * (double)a.vz[g] + a.kdt * acc.z;
-        a.vz[g] = (float)nvz;
+        const double nvz = a.ff * (double)a.vz[o] + a.kdt * acc.z;
+        a.ovz[g] = (float)nvz;
         a.oz[g] = integrate_axis(hz, nvz, a.dt, WRAP);
     }
+    a.otype[g] = htype;
+    a.oid[g] = a.id[o];
     if constexpr (DEBUG) {
         a.dbg_count[g] = acc.cnt;
         a.dbg_hash[g] = acc.hash;
@@ -215,17 +222,19 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
                 const int lo = (int)a.cell_start[ci], hi = (int)a.cell_end[ci];
                 for (int j = lo; j < hi; ++j) {
                     if (j == g) continue;
-                    const int dxi = (int)(a.sx[j] - hx), dyi = (int)(a.sy[j] - hy);
-                    const int dzi = DIM == 3 ? (int)(a.sz[j] - hz) : 0;
+                    const uint32_t jo = a.perm[j];
+                    const uint32_t qx = a.sx[jo], qy = a.sy[jo], qz = DIM == 3 ? a.sz[jo] : 0u;
+                    const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
+                    const int dzi = DIM == 3 ? (int)(qz - hz) : 0;
                     if (!WRAP) {
                         // plain difference of the clamped world: skip |D| >= 1/2
-                        bool ok = ((dxi >= 0) == (a.sx[j] >= hx)) & ((dyi >= 0) == (a.sy[j] >= hy));
-                        if (DIM == 3) ok &= ((dzi >= 0) == (a.sz[j] >= hz));
+                        bool ok = ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
+                        if (DIM == 3) ok &= ((dzi >= 0) == (qz >= hz));
                         if (!ok) continue;
                     }
-                    const int idx = row + a.type[j];
+                    const int idx = row + a.type[jo];
                     const double m = MLDS ? Mlds[idx] : a.matrix[idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[j] : 0u, a);
+                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, a);
                 }
             }
 }
@@ -299,8 +308,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const uint32_t hkey = a.skeys[g];
     const int hrow = (int)(hkey / (uint32_t)nx);
     const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
-    const uint32_t hx = a.sx[g], hy = a.sy[g], hz = DIM == 3 ? a.sz[g] : 0u;
-    const int mrow = a.type[g] * a.ntypes;
+    const uint32_t ho = a.perm[g];
+    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
+    const int htype = a.type[ho];
+    const int mrow = htype * a.ntypes;
 
     // parts = distinct rows among the homes (rows are non-decreasing with tid)
     const int prow = tid > 0 ? (int)(a.skeys[g - 1] / (uint32_t)nx) : -1;
@@ -387,7 +398,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                 if (off[mid] <= (uint32_t)s) lo = mid;
                 else hi = mid;
             }
-            const int q = (int)(gs[lo] + ((uint32_t)s - off[lo]));
+            const uint32_t q = a.perm[gs[lo] + ((uint32_t)s - off[lo])];
             sxy[s] = make_uint2(a.sx[q], a.sy[q]);
             if constexpr (DIM == 3) szz[s] = a.sz[q];
             if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
@@ -458,7 +469,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     } else if (active) {
         home_global<DIM, WRAP, DEBUG, MLDS>(a, M, g, hx, hy, hz, mrow, acc);
     }
-    if (active) finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, g);
+    if (active) finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, htype, ho, g);
 
     // interacting pairs, one 64-bit atomic per workgroup
     unsigned long long tot = active ? (unsigned long long)acc.cnt : 0ull;

@@ -6,7 +6,6 @@
 //                            (cell_end is the reference's `containers` array, Physics.java:329-348)
 //   k_radix_hist/_scatter    stable LSD radix sort of (key, index): the reference's
 //                            stable counting-sort permutation (Physics.java:342-353)
-//   k_gather                 SoA reorder into cell order (the array swap, Physics.java:350-353)
 //   k_force (plife_force.hpp) velocity phase + position phase fused (Physics.java:397-450);
 //                            legal because the update is Jacobi: velocities read only the
 //                            sorted step-start positions, new positions go to another buffer.
@@ -24,6 +23,9 @@ namespace plife {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+constexpr int kRadixTile = 4096;                 // elements per radix tile
+constexpr int kScatterBlock = 512;               // threads of the radix scatter
+constexpr int kRadixItems = kRadixTile / kScatterBlock;
 constexpr double kFix = 4294967296.0;       // 2^32
 constexpr double kInvFix = 1.0 / 4294967296.0;
 
@@ -146,6 +148,43 @@ __global__ __launch_bounds__(kBlock) void k_bin_keys(const uint32_t *__restrict_
     count_runs(key, valid, cell_count);
 }
 
+// Fused binning of one radix tile (kRadixTile elements in the radix kernels'
+// element order): cell keys, the cell histogram (run-aggregated atomics) and
+// this tile's histogram of the first radix digit.  Block 0 also clears the
+// pair counter of the step.
+template <int DIM, int D>
+__global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ px,
+                                                  const uint32_t *__restrict__ py,
+                                                  const uint32_t *__restrict__ pz, int n,
+                                                  double cs, int nx, int ntiles,
+                                                  uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ cell_count,
+                                                  uint32_t *__restrict__ hist,
+                                                  unsigned long long *__restrict__ pairs)
+{
+    constexpr int R = 1 << D;
+    __shared__ uint32_t h[R];
+    for (int t = threadIdx.x; t < R; t += 1024) h[t] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pairs = 0ull;
+    __syncthreads();
+    const int base = blockIdx.x * kRadixTile;
+#pragma unroll
+    for (int r = 0; r < kRadixTile / 1024; ++r) {
+        const int i = base + r * 1024 + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t key = 0xFFFFFFFFu;
+        if (valid) {
+            key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
+            if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
+            keys[i] = key;
+            atomicAdd(&h[key & (R - 1)], 1u);
+        }
+        count_runs(key, valid, cell_count);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R; t += 1024) hist[t * ntiles + blockIdx.x] = h[t];
+}
+
 // ---------------------------------------------------------------------------
 // device-wide scan of u32 (reduce -> scan block sums -> downsweep)
 // ---------------------------------------------------------------------------
@@ -218,11 +257,98 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, int n,
     }
 }
 
+// Two independent device-wide scans in the same three launches: blocks
+// [0, nb0) scan job 0, blocks [nb0, nb0 + nb1) scan job 1.  excl may alias in;
+// with zero_in (requires in != excl) the input is cleared once read, which
+// leaves the cell counters ready for the next step's histogram.
+struct ScanJob {
+    uint32_t *in;
+    int n;
+    uint32_t *excl;
+    uint32_t *incl;   // may be null
+    int zero_in;
+    uint32_t *bsum;   // ceil(n / kScanTile) block sums
+};
+
+__device__ __forceinline__ void scan_job_of(const ScanJob &j0, const ScanJob &j1, int nb0, int b,
+                                            ScanJob &j, int &blk)
+{
+    if (b < nb0) { j = j0; blk = b; }
+    else { j = j1; blk = b - nb0; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan2_reduce(ScanJob j0, ScanJob j1, int nb0)
+{
+    __shared__ uint32_t s_w[4];
+    ScanJob j;
+    int blk;
+    scan_job_of(j0, j1, nb0, blockIdx.x, j, blk);
+    const int base = blk * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < j.n) s += j.in[base + k];
+    uint32_t total;
+    block_excl_scan(s, s_w, total);
+    if (threadIdx.x == 0) j.bsum[blk] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan2_single(ScanJob j0, ScanJob j1)
+{
+    __shared__ uint32_t s_w[4];
+    const ScanJob j = blockIdx.x == 0 ? j0 : j1;
+    const int n = (j.n + kScanTile - 1) / kScanTile;
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += kScanTile) {
+        const int i0 = base + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (i0 + k < n) ? j.bsum[i0 + k] : 0u;
+            s += v[k];
+        }
+        uint32_t total;
+        uint32_t off = block_excl_scan(s, s_w, total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < n) j.bsum[i0 + k] = off;
+            off += v[k];
+        }
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan2_down(ScanJob j0, ScanJob j1, int nb0)
+{
+    __shared__ uint32_t s_w[4];
+    ScanJob j;
+    int blk;
+    scan_job_of(j0, j1, nb0, blockIdx.x, j, blk);
+    const int i0 = blk * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (i0 + k < j.n) ? j.in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(s, s_w, total) + j.bsum[blk];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (i0 + k < j.n) {
+            if (j.zero_in) j.in[i0 + k] = 0u;
+            j.excl[i0 + k] = off;
+            off += v[k];
+            if (j.incl) j.incl[i0 + k] = off;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // stable LSD radix sort of (key, value) -- reduce-then-scan per digit pass
 // ---------------------------------------------------------------------------
-constexpr int kRadixItems = 16;
-constexpr int kRadixTile = kBlock * kRadixItems;
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys, int n,
@@ -235,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
     __syncthreads();
     const int base = blockIdx.x * kRadixTile;
 #pragma unroll 4
-    for (int k = 0; k < kRadixItems; ++k) {
+    for (int k = 0; k < kRadixTile / kBlock; ++k) {
         const int i = base + k * kBlock + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
     }
@@ -258,31 +384,27 @@ struct AuxBuf {
 // base + r*256 + tid, and inside a round waves go in order and lanes go in
 // order (wave-level match by digit ballots).  gofs is the digit-major
 // exclusive scan of the per-tile histograms.  vin == nullptr means value = index.
-//
-// GATHER (last pass): besides the permutation, copy the particle SoA from its
-// pre-sort slot to its sorted slot (the array swap of Physics.java:350-353).
-template <int D, int GATHER_DIM>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__restrict__ kin,
-                                                          const uint32_t *__restrict__ vin,
-                                                          uint32_t *__restrict__ kout,
-                                                          uint32_t *__restrict__ vout,
-                                                          const uint32_t *__restrict__ gofs,
-                                                          int n, int shift, int ntiles,
-                                                          PosBuf ps, AuxBuf as, PosBuf pd, AuxBuf ad)
+template <int D>
+__global__ __launch_bounds__(kScatterBlock) void k_radix_scatter(const uint32_t *__restrict__ kin,
+                                                                 const uint32_t *__restrict__ vin,
+                                                                 uint32_t *__restrict__ kout,
+                                                                 uint32_t *__restrict__ vout,
+                                                                 const uint32_t *__restrict__ gofs,
+                                                                 int n, int shift, int ntiles)
 {
     constexpr int R = 1 << D;
-    constexpr int NW = kBlock / kWave;
+    constexpr int NW = kScatterBlock / kWave;
     __shared__ uint32_t s_k[kRadixTile];
     __shared__ uint32_t s_v[kRadixTile];
     __shared__ uint32_t s_cnt[NW][R];
     __shared__ uint32_t s_off[NW][R];
     __shared__ uint32_t s_start[R];
     __shared__ uint32_t s_gb[R];
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[NW];
 
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int base = blockIdx.x * kRadixTile;
-    for (int t = tid; t < NW * R; t += kBlock) (&s_cnt[0][0])[t] = 0;
+    for (int t = tid; t < NW * R; t += kScatterBlock) (&s_cnt[0][0])[t] = 0;
     uint32_t running = 0;
     uint32_t key[kRadixItems], val[kRadixItems], rank[kRadixItems];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -290,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
 
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
-        const int i = base + r * kBlock + tid;
+        const int i = base + r * kScatterBlock + tid;
         const bool valid = i < n;
         const uint32_t k = valid ? kin[i] : 0u;
         const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
@@ -321,8 +443,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
         rank[r] = valid ? s_off[w][d] + lr : 0u;
     }
 
-    uint32_t total;
-    const uint32_t st = block_excl_scan(tid < R ? running : 0u, s_w, total);
+    // exclusive scan of the tile's digit counts (threads tid < R hold them)
+    const uint32_t rv = tid < R ? running : 0u;
+    const uint32_t rinc = wave_incl_scan(rv);
+    if (lane == 63) s_w[w] = rinc;
+    __syncthreads();
+    uint32_t wb = 0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) wb += ww < w ? s_w[ww] : 0u;
+    const uint32_t st = wb + rinc - rv;
     if (tid < R) {
         s_start[tid] = st;
         s_gb[tid] = gofs[tid * ntiles + blockIdx.x];
@@ -330,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
-        const int i = base + r * kBlock + tid;
+        const int i = base + r * kScatterBlock + tid;
         if (i < n) {
             const uint32_t d = (key[r] >> shift) & (R - 1);
             const uint32_t lp = s_start[d] + rank[r];
@@ -340,45 +469,13 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
     }
     __syncthreads();
     const int nvalid = min(kRadixTile, n - base);
-    for (int j = tid; j < nvalid; j += kBlock) {
+    for (int j = tid; j < nvalid; j += kScatterBlock) {
         const uint32_t k = s_k[j];
         const uint32_t d = (k >> shift) & (R - 1);
         const uint32_t g = s_gb[d] + (uint32_t)j - s_start[d];
-        const uint32_t v = s_v[j];
         if (kout) kout[g] = k;
-        vout[g] = v;
-        if constexpr (GATHER_DIM != 0) {
-            pd.x[g] = ps.x[v];
-            pd.y[g] = ps.y[v];
-            ad.vx[g] = as.vx[v];
-            ad.vy[g] = as.vy[v];
-            if constexpr (GATHER_DIM == 3) {
-                pd.z[g] = ps.z[v];
-                ad.vz[g] = as.vz[v];
-            }
-            ad.type[g] = as.type[v];
-            ad.id[g] = as.id[v];
-        }
+        vout[g] = s_v[j];
     }
-}
-
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_gather(int n, const uint32_t *__restrict__ perm,
-                                                   PosBuf ps, AuxBuf as, PosBuf pd, AuxBuf ad)
-{
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t o = perm[s];
-    pd.x[s] = ps.x[o];
-    pd.y[s] = ps.y[o];
-    ad.vx[s] = as.vx[o];
-    ad.vy[s] = as.vy[o];
-    if constexpr (DIM == 3) {
-        pd.z[s] = ps.z[o];
-        ad.vz[s] = as.vz[o];
-    }
-    ad.type[s] = as.type[o];
-    ad.id[s] = as.id[o];
 }
 
 }  // namespace plife
