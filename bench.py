@@ -44,7 +44,22 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="instead of the benchmark, run 5 plain device copies of BYTES bytes "
+                        "(a known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration(nbytes: int) -> int:
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
+    return 0
 
 
 def cpu_threads(requested: int) -> int:
@@ -87,6 +102,8 @@ def cpu_baseline(world, steps: int, threads: int) -> dict:
 
 def main() -> int:
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the world is stepped "
               "on a single MI355X and multi-GPU sharding is not part of this build.",
