@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark of the particle-life physics step (Physics.update()) on one MI355X.
 
-Workload: the single large 2D world -- N = 1,000,000 particles, 8 types, rmax 0.01
-in [-1,1)^2 (0.005 in the reference's unit box, ~78.5 interacting neighbours),
-beta 0.3, friction half-life 0.04 s, dt 0.01 s, periodic.  One step = one
-Physics.update(): cell keys, stable sort, gather, fused force + integrate.
+Workload (default, --config c2): the single large 2D world -- N = 1,000,000
+particles, 8 types, rmax 0.01 in [-1,1)^2 (0.005 in the reference's unit box,
+~78.5 interacting neighbours), beta 0.3, friction half-life 0.04 s, dt 0.01 s,
+periodic.  One step = one Physics.update(): cell keys, stable sort, fused
+force + integrate.  --config c1 / c3 run the 10k 2D and clustered 4M 3D worlds.
 
 Prints ONE JSON line.  `value` is pair-interactions/s (ordered interacting
 pairs per step, counted exactly by the kernel, times steps/s) with the state
@@ -12,7 +13,7 @@ resident in HBM; the timed region is K steps between two HIP events on the
 library's stream, bracketed by synchronisations.
 
 roofline: the force+integrate kernel (the dominant one) against the FP32
-vector peak, with algorithmic work of 22 FLOP per interacting 2D pair.
+vector peak, with algorithmic work of 22 FLOP per interacting 2D pair (28 in 3D).
 cpu_baseline: the FP64 C restatement (oracle/) on this host's cores, a bounded
 sample of the same world.
 """
@@ -28,8 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "particle-life-app_amd"))
 sys.path.insert(0, ROOT)
 
-FLOP_PER_PAIR_2D = 22          # algorithmic FLOPs per interacting pair (2D)
-BYTES_PER_PARTICLE_2D = 36     # read x,y,vx,vy,type + write x,y,vx,vy
+FLOP_PER_PAIR = {2: 22, 3: 28}          # algorithmic FLOPs per interacting pair
+BYTES_PER_PARTICLE = {2: 36, 3: 52}      # read x,v,type + write x,v
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 
@@ -39,7 +40,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--config", choices=("c1", "c2", "c3"), default="c2",
+                   help="c2 (default): the 1M-particle 2D world the metric is quoted on; "
+                        "c1: the 10k 2D world; c3: the clustered 4M 3D world")
+    p.add_argument("--n", type=int, default=0, help="particle count override (0 = the config's)")
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, available cores)")
@@ -112,7 +116,8 @@ def main() -> int:
 
     from particle_life import Engine, worlds
 
-    w = worlds.config2(args.n)
+    make = {"c1": worlds.config1, "c2": worlds.config2, "c3": worlds.config3}[args.config]
+    w = make(args.n) if args.n else make()
     eng = Engine(0)
     eng.load_world(w)
     eng.step(args.warmup)
@@ -133,16 +138,25 @@ def main() -> int:
     value = pairs_last * steps_per_s
 
     force_s = prof["force_ms"] / 1e3
-    achieved_tflops = FLOP_PER_PAIR_2D * pairs / force_s / 1e12
+    achieved_tflops = FLOP_PER_PAIR[w.dim] * pairs / force_s / 1e12
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "force_c2_traffic.json")
-    if os.path.exists(tpath):
+    if args.config == "c2" and not args.n and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    workloads = {
+        "c1": "plain 2D world: N=10,000, 6 types, rmax 0.04 in [-1,1)^2 (0.02 unit box), beta 0.3, "
+              "friction half-life 0.04 s, dt 0.02 s, wrap",
+        "c2": "single large 2D world: N=1,000,000, 8 types, rmax 0.01 in [-1,1)^2 "
+              "(0.005 unit box), beta 0.3, friction half-life 0.04 s, dt 0.01 s, wrap",
+        "c3": "clustered 3D world: N=4,000,000, 12 types, 64 Gaussian blobs (sigma 0.03 in "
+              "[-1,1)^3), rmax 0.02 (0.01 unit box), beta 0.25, dt 0.01 s, wrap",
+    }
     out = {
-        "metric": "pair-interactions/s & steps/s at N=1M 2D",
+        "metric": "pair-interactions/s & steps/s at N=1M 2D" if args.config == "c2"
+                  else f"pair-interactions/s & steps/s ({args.config})",
         "value": value,
         "unit": "pair-interactions/s",
         "n_gpus": 1,
@@ -159,21 +173,20 @@ def main() -> int:
         "dtype": "f32 state, f64 force law, u32 fixed-point positions",
         "data": "synthetic (seeded SplitMix64 world)",
         "config": {
-            "workload": "single large 2D world: N=1,000,000, 8 types, rmax 0.01 in [-1,1)^2 "
-                        "(0.005 unit box), beta 0.3, friction half-life 0.04 s, dt 0.01 s, wrap",
-            "n": w.n, "ntypes": w.ntypes, "rmax_unit_box": w.rmax, "dt": w.dt,
+            "workload": workloads[args.config],
+            "n": w.n, "dim": w.dim, "ntypes": w.ntypes, "rmax_unit_box": w.rmax, "dt": w.dt,
         },
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
         "roofline": {
-            "kernel": "k_force<2,true,false>",
+            "kernel": f"k_force<{w.dim}, wrap, no-debug>",
             "bound": "valu",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "algorithmic": f"{FLOP_PER_PAIR_2D} FLOP/pair x {pairs} pairs per launch",
-            "hbm_frac_algorithmic": BYTES_PER_PARTICLE_2D * w.n * steps_per_s / 1e9 / PEAK_HBM_GBS,
+            "algorithmic": f"{FLOP_PER_PAIR[w.dim]} FLOP/pair x {pairs} pairs per launch",
+            "hbm_frac_algorithmic": BYTES_PER_PARTICLE[w.dim] * w.n * steps_per_s / 1e9 / PEAK_HBM_GBS,
         },
     }
     if not args.no_cpu:

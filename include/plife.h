@@ -129,7 +129,10 @@ int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint
  * candidate-list entries of the force kernel; stage_slack in [0, 16]: LDS
  * staging capacity as a multiple of the expected staged particles;
  * direct_launch 1: launch every kernel directly instead of replaying a HIP
- * graph of two steps.  Results do not depend on the tuning. */
+ * graph of two steps.  Results do not depend on the tuning.  Automatic
+ * fields follow the world's clustering: every step measures the mean size of
+ * the cell a particle sits in, and each plife_update plans its launches from
+ * the latest measurement that has reached the host. */
 typedef struct {
     int32_t block_homes;
     int32_t list_cap;

@@ -51,9 +51,14 @@ int fail(int code, const char *fmt, ...)
 
 constexpr int kMaxTypes = 256;        // the app caps the matrix size at 256 (app/Main.java:745)
 constexpr int kMaxTypesLds = 32;      // matrix staged in LDS up to 32x32 (8 KB of FP64)
-constexpr int kListCap = 32;          // per-thread candidate list entries (u16) in LDS
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxLds = 65536;
+constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
+
+struct ForcePlan {
+    int bs, lcap, max_cells, cap;
+    size_t lds;
+};
 
 template <typename T>
 int dev_alloc(T **p, size_t count)
@@ -106,7 +111,10 @@ struct plife_physics {
     uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_end = nullptr;
     size_t hist_cap = 0, bsum_cap = 0;
     uint32_t *hist = nullptr, *bsum = nullptr;
-    unsigned long long *d_pairs = nullptr;
+    unsigned long long *d_pairs = nullptr;   // [0] interacting pairs, [1] sum of cell counts^2
+    uint64_t *stat_host = nullptr;           // pinned copy of d_pairs[1], written every step
+    double occupancy = 0.0;                  // particle-weighted cell occupancy seen by the planner
+    bool clustered = false;                  // the planner's regime (with hysteresis)
 
     // grid of the last step
     bool stepped = false;
@@ -129,7 +137,7 @@ struct plife_physics {
     // launch tuning (0 = automatic) and what the last force launch used
     plife_tuning tune{};
     size_t last_lds = 0;
-    int last_cap = 0;
+    int last_cap = 0, last_bs = 0, last_lcap = 0;
 
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
@@ -231,20 +239,6 @@ int ensure_debug(plife_physics *h)
     if ((rc = dev_alloc(&h->dbg_count, cap))) return rc;
     if ((rc = dev_alloc(&h->dbg_hash, cap))) return rc;
     h->dbg_cap = cap;
-    return PLIFE_OK;
-}
-
-// device-wide scan: excl[i] = sum in[0..i), incl[i] = sum in[0..i]; in may alias excl.
-int scan_u32(plife_physics *h, const uint32_t *in, int n, uint32_t *excl, uint32_t *incl)
-{
-    if (n <= 0) return PLIFE_OK;
-    const int nb = ceil_div(n, kScanTile);
-    int rc = ensure_u32(&h->bsum, &h->bsum_cap, (size_t)nb);
-    if (rc) return rc;
-    k_scan_reduce<<<nb, kBlock, 0, h->stream>>>(in, n, h->bsum);
-    k_scan_single<<<1, kBlock, 0, h->stream>>>(h->bsum, nb);
-    k_scan_down<<<nb, kBlock, 0, h->stream>>>(in, n, h->bsum, excl, incl);
-    LAUNCH_CHECK();
     return PLIFE_OK;
 }
 
@@ -412,6 +406,74 @@ int record(plife_physics *h, PhaseEvents *ev, int k)
     return PLIFE_OK;
 }
 
+// Launch shape of the force kernel for the current world.  Explicit tuning
+// fields win.  Otherwise the planner reads the particle-weighted cell
+// occupancy (sum over cells of count^2, divided by N: the mean size of the
+// cell a particle sits in) measured by an earlier step:
+//   * uniform worlds: 256 homes per workgroup and a staging buffer that holds
+//     the whole neighbourhood (one window), list capacity rising with density;
+//   * clustered worlds (occupancy far above the mean N / cells): 128 homes per
+//     workgroup and a small staging buffer, so that many workgroups share a CU
+//     while the dense neighbourhoods stream through it in windows.
+// The plan never changes results: every home visits its candidates in the
+// same order whatever the windows and workgroup size.
+ForcePlan plan_force(const plife_physics *h, int dim, long long ncells, int n)
+{
+    const int NR = dim == 3 ? 9 : 3;
+    const double avg = (double)n / (double)std::max(ncells, 1LL);
+    const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
+    const bool clustered = h->clustered;
+    const plife_tuning &t = h->tune;
+    ForcePlan p;
+    p.bs = t.block_homes ? t.block_homes : (clustered ? 128 : 256);
+    p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))
+                            : (clustered ? 32 : occ >= 12.0 ? 24 : 16);
+    p.lcap = std::min(p.lcap, 16384 / p.bs);             // lists stay within 32 KB of LDS
+    // a workgroup of bs homes spans ~bs/avg + 1 cells of a row; it stages NR
+    // rows of (span + 2) cells.  Clustered worlds keep a short cell table: the
+    // few sparse workgroups that span more cells take the global path.
+    const double span = p.bs / std::max(avg, 1e-9) + 1.0;
+    p.max_cells = (int)std::min<double>(std::max(64.0, std::ceil(NR * (span + 2.0) * 1.5 + 16)),
+                                        clustered ? 384.0 : 2048.0);
+    long long want;
+    size_t budget = kMaxLds;
+    if (t.stage_slack > 0.0 || !clustered) {
+        const double slack = t.stage_slack > 0.0 ? t.stage_slack : 1.25;
+        want = (long long)std::ceil(NR * (span + 2.0) * avg * slack) + 64;
+    } else {
+        want = 65535;
+        budget = kClusteredLds;
+    }
+    want = std::min<long long>((want + 63) / 64 * 64, 65535);
+    const bool dbg = h->debug != 0, m_in_lds = h->ntypes <= kMaxTypesLds;
+    auto total = [&](long long c) {
+        return force_lds_layout(dim, dbg, h->ntypes, m_in_lds, (int)c, p.lcap, p.bs, p.max_cells).total;
+    };
+    // keep room for a useful staging window: shrink the cell table first
+    const long long min_cap = std::min<long long>(want, 256);
+    while (p.max_cells > 64 && total(min_cap) > budget) p.max_cells = std::max(64, p.max_cells * 3 / 4);
+    // then shrink the staging capacity until the layout fits the LDS budget
+    while (want > 64 && total(want) > budget) want -= 64;
+    p.cap = (int)std::max(0LL, want);
+    p.lds = force_lds_layout(dim, dbg, h->ntypes, m_in_lds, p.cap, p.lcap, p.bs, p.max_cells).total;
+    return p;
+}
+
+// Read the clustering measure of the latest step the host has seen complete
+// (a pinned copy refreshed by every step; a stale value only affects speed).
+void refresh_occupancy(plife_physics *h)
+{
+    const uint64_t q = __atomic_load_n(h->stat_host, __ATOMIC_RELAXED);
+    if (q == 0 || h->n <= 0) return;
+    h->occupancy = (double)q / (double)h->n;
+    const double avg = (double)h->n / (double)std::max(grid_of(h->st).ncells, 1LL);
+    // clustered: cells seen by a particle hold far more than the average (a
+    // Poisson world has occupancy avg + 1) and enough to fill long windows
+    const double ratio = h->occupancy / (avg + 1.0);
+    if (!h->clustered && ratio > 4.0 && h->occupancy > 64.0) h->clustered = true;
+    else if (h->clustered && (ratio < 3.0 || h->occupancy < 48.0)) h->clustered = false;
+}
+
 int do_step(plife_physics *h, PhaseEvents *ev)
 {
     const plife_settings &s = h->st;
@@ -439,13 +501,17 @@ int do_step(plife_physics *h, PhaseEvents *ev)
         if (s.dim == 3) launch_bin<3>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
         else launch_bin<2>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
     } else {
-        k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 1);
+        k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 2);
     }
     LAUNCH_CHECK();
-    if ((rc = scan_jobs(h, ScanJob{h->cell_count, nc, h->cell_start, h->cell_end, 1, nullptr},
+    if ((rc = scan_jobs(h, ScanJob{h->cell_count, nc, h->cell_start, h->cell_end, 1, nullptr,
+                                   h->d_pairs + 1},
                         n > 0 ? ScanJob{h->hist, (int)plan.hist_len, h->hist, nullptr, 0, nullptr}
                               : no_job())))
         return rc;
+    // the clustering measure travels to pinned host memory for the launch planner
+    HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs + 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           h->stream));
     h->cells_dirty = false;
     if ((rc = record(h, ev, 1))) return rc;
 
@@ -494,26 +560,14 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.ntypes = h->ntypes;
     a.n = n;
     a.nx = gr.nx;
-    const double avg = (double)n / (double)gr.ncells;
-    const int NR = s.dim == 3 ? 9 : 3;
+    const ForcePlan fp = plan_force(h, s.dim, gr.ncells, n);
     const bool m_in_lds = h->ntypes <= kMaxTypesLds;
     a.m_in_lds = m_in_lds ? 1 : 0;
     a.skeys = h->skeys;
-    a.lcap = h->tune.list_cap > 0 ? std::max(8, std::min(h->tune.list_cap, 512)) : kListCap;
-    // a workgroup of BS homes spans ~BS/avg + 1 cells of a row; it stages NR
-    // rows of (span + 2) cells
-    const int bs = h->tune.block_homes == 64 || h->tune.block_homes == 128 ? h->tune.block_homes : 256;
-    const double slack = h->tune.stage_slack > 0 ? h->tune.stage_slack : 1.25;
-    const double span = bs / std::max(avg, 1e-9) + 1.0;
-    a.max_cells = (int)std::min<double>(std::max(64.0, std::ceil(NR * (span + 2.0) * 1.5 + 16)), 2048.0);
-    long long want = (long long)std::ceil(NR * (span + 2.0) * avg * slack) + 64;
-    want = std::min<long long>((want + 63) / 64 * 64, 65535);
-    // shrink the staging capacity until the layout fits the LDS budget
-    const bool dbg = h->debug != 0;
-    while (want > 0 && force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, (int)want, a.lcap, bs,
-                                        a.max_cells).total > (size_t)kMaxLds)
-        want -= 64;
-    a.cap = (int)std::max(0LL, want);
+    a.lcap = fp.lcap;
+    a.max_cells = fp.max_cells;
+    a.cap = fp.cap;
+    const int bs = fp.bs;
     a.all_global = gr.nx < 3 ? 1 : 0;
     a.r2hi = (float)(R2 * (1.0 + 0x1p-20));
     a.r2u = (s.rmax * s.rmax) * (kFix * kFix);          // Physics.java:432, exact 2^64 scaling
@@ -530,10 +584,11 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.pair_counter = h->d_pairs;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
-    const size_t lds = force_lds_layout(s.dim, dbg, h->ntypes, m_in_lds, a.cap, a.lcap, bs,
-                                        a.max_cells).total;
+    const size_t lds = fp.lds;
     h->last_lds = lds;
     h->last_cap = a.cap;
+    h->last_bs = bs;
+    h->last_lcap = a.lcap;
     if ((rc = launch_force_any(h, a, bs, lds))) return rc;
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
@@ -562,6 +617,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
         int n, ntypes, debug, pc, ac;
         plife_settings st;
         plife_tuning tune;
+        ForcePlan plan;
         const void *ptr[32];
     } k;
     std::memset(&k, 0, sizeof k);
@@ -572,6 +628,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     k.ac = h->ac;
     k.st = h->st;
     k.tune = h->tune;
+    k.plan = plan_force(h, h->st.dim, grid_of(h->st).ncells, h->n);
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
                        h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
@@ -617,6 +674,7 @@ int capture_two_steps(plife_physics *h)
 int run_steps(plife_physics *h, int nsteps, bool honour_stop)
 {
     int k = 0, rc;
+    refresh_occupancy(h);
     if (nsteps >= 2 && !h->tune.direct_launch) {
         if ((rc = prepare_step(h))) return rc;
         std::vector<unsigned char> key = graph_key(h);
@@ -679,7 +737,12 @@ int plife_create(int32_t device, plife_physics **out)
         hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
         if (e != hipSuccess) rc = fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    if (!rc) rc = dev_alloc(&h->d_pairs, 1);
+    if (!rc) rc = dev_alloc(&h->d_pairs, 2);
+    if (!rc) {
+        hipError_t e = hipHostMalloc((void **)&h->stat_host, sizeof(uint64_t), hipHostMallocDefault);
+        if (e != hipSuccess) rc = fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
+        else *h->stat_host = 0;
+    }
     if (!rc) rc = dev_alloc(&h->d_matrix, kMaxTypes * kMaxTypes);
     if (rc) {
         plife_destroy(h);
@@ -703,6 +766,7 @@ int plife_destroy(plife_physics *h)
     dev_free(h->hist);
     dev_free(h->bsum);
     dev_free(h->d_pairs);
+    if (h->stat_host) (void)hipHostFree(h->stat_host);
     dev_free(h->d_matrix);
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
@@ -785,6 +849,10 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
         HIP_TRY(hipMemcpyAsync(h->id[0], ids.data(), n * 4, hipMemcpyHostToDevice, h->stream));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
+    // a new world: forget the clustering measure of the old one
+    *h->stat_host = 0;
+    h->occupancy = 0.0;
+    h->clustered = false;
     return PLIFE_OK;
 }
 
@@ -914,8 +982,9 @@ int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
 {
     if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     *t = h->tune;
-    if (t->list_cap == 0) t->list_cap = kListCap;
-    if (t->block_homes == 0) t->block_homes = 256;
+    // automatic fields report what the last force launch used
+    if (t->list_cap == 0) t->list_cap = h->last_lcap;
+    if (t->block_homes == 0) t->block_homes = h->last_bs;
     if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
     return PLIFE_OK;
 }

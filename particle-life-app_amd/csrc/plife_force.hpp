@@ -243,8 +243,9 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
 // of the cell-sorted order, so every wave is full.  The homes span a few
 // consecutive cells of one grid row, or (at row ends) of a few rows: each
 // distinct row is a "part" whose 3^(DIM-1) stencil rows x (span + 2) cells are
-// staged in LDS once for the whole workgroup.  Workgroups with more parts or
-// staged cells than the LDS budget take the global path.
+// staged in LDS for the whole workgroup, in windows of at most `cap` particles
+// when the neighbourhood is denser than the staging buffer.  Workgroups with
+// more parts or staged cells than the cell table holds take the global path.
 constexpr int kMaxParts = 4;
 
 struct ForceLds {
@@ -347,9 +348,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         p_base[p + 1] = p_base[p] + NR * p_w[p];
     }
     const int NC = p_base[kMaxParts];
-    bool use_lds = !a.all_global && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
+    bool use_lds = !a.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
 
-    // staged cells: start and count, then an exclusive prefix in LDS
+    // staged cells: start and count, then an exclusive prefix in LDS (carry ends
+    // as the staged particle count, the same in every thread)
     uint32_t carry = 0;
     if (use_lds) {
         for (int f0 = 0; f0 < NC; f0 += BS) {
@@ -385,25 +387,6 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             __syncthreads();
         }
         if (tid == 0) off[NC] = carry;
-        use_lds = (int)carry <= a.cap;
-    }
-    __syncthreads();
-
-    const int S = (int)carry;
-    if (use_lds) {
-        for (int s = tid; s < S; s += BS) {
-            int lo = 0, hi = NC;               // off[lo] <= s < off[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (off[mid] <= (uint32_t)s) lo = mid;
-                else hi = mid;
-            }
-            const uint32_t q = a.perm[gs[lo] + ((uint32_t)s - off[lo])];
-            sxy[s] = make_uint2(a.sx[q], a.sy[q]);
-            if constexpr (DIM == 3) szz[s] = a.sz[q];
-            if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
-            st[s] = (uint8_t)a.type[q];
-        }
     }
     __syncthreads();
 
@@ -427,45 +410,70 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, a.cs) == nx;
         const int j_lo = jh - 1 + (q2x ? 1 : 0);
 
-        int cnt = 0;
-        auto flush = [&]() {
-            for (int j = 0; j < cnt; ++j) {
-                const int k = lists[j * BS + tid];
-                const uint2 q = sxy[k];
-                const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
-                const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
-                const int idx = mrow + (int)st[k];
-                const double m = MLDS ? M[idx] : a.matrix[idx];
-                pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
+        // The staged neighbourhood is S particles; it is brought into LDS in
+        // windows of at most cap (one window for ordinary densities, several
+        // for clustered cells) and every lane scans its ranges clipped to it.
+        const int S = (int)carry;
+        for (int w0 = 0; w0 < S; w0 += a.cap) {
+            const int wn = min(a.cap, S - w0);
+            for (int s = tid; s < wn; s += BS) {
+                const int sg = w0 + s;
+                int lo = 0, hi = NC;           // off[lo] <= sg < off[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (off[mid] <= (uint32_t)sg) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t q = a.perm[gs[lo] + ((uint32_t)sg - off[lo])];
+                sxy[s] = make_uint2(a.sx[q], a.sy[q]);
+                if constexpr (DIM == 3) szz[s] = a.sz[q];
+                if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
+                st[s] = (uint8_t)a.type[q];
             }
-            cnt = 0;
-        };
+            __syncthreads();
+
+            int cnt = 0;
+            auto flush = [&]() {
+                for (int j = 0; j < cnt; ++j) {
+                    const int k = lists[j * BS + tid];
+                    const uint2 q = sxy[k];
+                    const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
+                    const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
+                    const int idx = mrow + (int)st[k];
+                    const double m = MLDS ? M[idx] : a.matrix[idx];
+                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
+                }
+                cnt = 0;
+            };
 #pragma unroll 1
-        for (int r = 0; r < NR; ++r) {
-            const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-            const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
-            const int rb = pb + r * pw;
-            const int klo = (int)off[rb + j_lo];
-            const int len = skip ? 0 : (int)off[rb + jh + 2] - klo;
-            const int K = wave_max(len);
-            for (int i = 0; i < K; i += U) {
-                if (__any(cnt > a.lcap - U)) flush();
+            for (int r = 0; r < NR; ++r) {
+                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
+                const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
+                const int rb = pb + r * pw;
+                const int klo = max((int)off[rb + j_lo], w0) - w0;
+                const int khi = min((int)off[rb + jh + 2], w0 + wn) - w0;
+                const int len = skip ? 0 : max(0, khi - klo);
+                const int K = wave_max(len);
+                for (int i = 0; i < K; i += U) {
+                    if (__any(cnt > a.lcap - U)) flush();
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int ii = i + u;
-                    if (ii < len) {
-                        const int k = klo + ii;
-                        const uint2 q = sxy[k];
-                        const uint32_t qz = DIM == 3 ? szz[k] : 0u;
-                        if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
-                            lists[cnt * BS + tid] = (uint16_t)k;
-                            ++cnt;
+                    for (int u = 0; u < U; ++u) {
+                        const int ii = i + u;
+                        if (ii < len) {
+                            const int k = klo + ii;
+                            const uint2 q = sxy[k];
+                            const uint32_t qz = DIM == 3 ? szz[k] : 0u;
+                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
+                                lists[cnt * BS + tid] = (uint16_t)k;
+                                ++cnt;
+                            }
                         }
                     }
                 }
             }
+            flush();
+            __syncthreads();
         }
-        flush();
     } else if (active) {
         home_global<DIM, WRAP, DEBUG, MLDS>(a, M, g, hx, hy, hz, mrow, acc);
     }

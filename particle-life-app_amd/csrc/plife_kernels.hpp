@@ -1,8 +1,9 @@
 // plife_kernels.hpp -- gfx950 kernels of the particle-life physics step.
 //
 // One step (reference: backend/Physics.java:112-134):
-//   k_fill / k_bin_keys      cell key per particle (Physics.java:362-375) + cell histogram
-//   k_scan_*                 exclusive/inclusive scan -> cell_start / cell_end
+//   k_bin_tile               cell key per particle (Physics.java:362-375), cell histogram,
+//                            first radix-digit histogram
+//   k_scan2_*                exclusive/inclusive scans -> cell_start / cell_end
 //                            (cell_end is the reference's `containers` array, Physics.java:329-348)
 //   k_radix_hist/_scatter    stable LSD radix sort of (key, index): the reference's
 //                            stable counting-sort permutation (Physics.java:342-353)
@@ -129,25 +130,6 @@ __device__ __forceinline__ void count_runs(uint32_t key, bool valid, uint32_t *_
     }
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_bin_keys(const uint32_t *__restrict__ px,
-                                                     const uint32_t *__restrict__ py,
-                                                     const uint32_t *__restrict__ pz, int n,
-                                                     double cs, int nx,
-                                                     uint32_t *__restrict__ keys,
-                                                     uint32_t *__restrict__ cell_count)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t key = 0xFFFFFFFFu;
-    if (valid) {
-        key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
-        if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
-        keys[i] = key;
-    }
-    count_runs(key, valid, cell_count);
-}
-
 // Fused binning of one radix tile (kRadixTile elements in the radix kernels'
 // element order): cell keys, the cell histogram (run-aggregated atomics) and
 // this tile's histogram of the first radix digit.  Block 0 also clears the
@@ -165,7 +147,7 @@ __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ 
     constexpr int R = 1 << D;
     __shared__ uint32_t h[R];
     for (int t = threadIdx.x; t < R; t += 1024) h[t] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *pairs = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < 2) pairs[threadIdx.x] = 0ull;   // pair count, sum of squares
     __syncthreads();
     const int base = blockIdx.x * kRadixTile;
 #pragma unroll
@@ -191,76 +173,12 @@ __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ 
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in, int n,
-                                                        uint32_t *__restrict__ bsum)
-{
-    __shared__ uint32_t s_w[4];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += in[base + k];
-    uint32_t total;
-    block_excl_scan(s, s_w, total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// In-place exclusive scan of a[0..n) by one block (n = number of tiles).
-__global__ __launch_bounds__(kBlock) void k_scan_single(uint32_t *__restrict__ a, int n)
-{
-    __shared__ uint32_t s_w[4];
-    uint32_t carry = 0;
-    for (int base = 0; base < n; base += kScanTile) {
-        const int i0 = base + threadIdx.x * kScanItems;
-        uint32_t v[kScanItems];
-        uint32_t s = 0;
-#pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            v[k] = (i0 + k < n) ? a[i0 + k] : 0u;
-            s += v[k];
-        }
-        uint32_t total;
-        uint32_t off = block_excl_scan(s, s_w, total) + carry;
-#pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            if (i0 + k < n) a[i0 + k] = off;
-            off += v[k];
-        }
-        carry += total;
-    }
-}
-
-// `in` may alias `excl` (in-place scan): no __restrict__ on those two.
-__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, int n,
-                                                      const uint32_t *__restrict__ boff,
-                                                      uint32_t *excl,
-                                                      uint32_t *__restrict__ incl)
-{
-    __shared__ uint32_t s_w[4];
-    const int i0 = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
-        s += v[k];
-    }
-    uint32_t total;
-    uint32_t off = block_excl_scan(s, s_w, total) + boff[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (i0 + k < n) {
-            if (excl) excl[i0 + k] = off;
-            off += v[k];
-            if (incl) incl[i0 + k] = off;
-        }
-    }
-}
-
 // Two independent device-wide scans in the same three launches: blocks
 // [0, nb0) scan job 0, blocks [nb0, nb0 + nb1) scan job 1.  excl may alias in;
 // with zero_in (requires in != excl) the input is cleared once read, which
-// leaves the cell counters ready for the next step's histogram.
+// leaves the cell counters ready for the next step's histogram.  With sq,
+// the reduce pass also adds the sum of squared inputs to *sq (for cell
+// counts: the number of (particle, cell-mate) pairs, a clustering measure).
 struct ScanJob {
     uint32_t *in;
     int n;
@@ -268,6 +186,7 @@ struct ScanJob {
     uint32_t *incl;   // may be null
     int zero_in;
     uint32_t *bsum;   // ceil(n / kScanTile) block sums
+    unsigned long long *sq;   // may be null
 };
 
 __device__ __forceinline__ void scan_job_of(const ScanJob &j0, const ScanJob &j1, int nb0, int b,
@@ -280,17 +199,27 @@ __device__ __forceinline__ void scan_job_of(const ScanJob &j0, const ScanJob &j1
 __global__ __launch_bounds__(kBlock) void k_scan2_reduce(ScanJob j0, ScanJob j1, int nb0)
 {
     __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_q[kBlock / kWave];
     ScanJob j;
     int blk;
     scan_job_of(j0, j1, nb0, blockIdx.x, j, blk);
     const int base = blk * kScanTile + threadIdx.x * kScanItems;
     uint32_t s = 0;
+    unsigned long long q = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
-        if (base + k < j.n) s += j.in[base + k];
+        if (base + k < j.n) {
+            const uint32_t v = j.in[base + k];
+            s += v;
+            q += (unsigned long long)v * v;
+        }
     uint32_t total;
     block_excl_scan(s, s_w, total);
     if (threadIdx.x == 0) j.bsum[blk] = total;
+    if (j.sq) {                                    // uniform per block
+        q = block_sum_u64(q, s_q);
+        if (threadIdx.x == 0 && q) atomicAdd(j.sq, q);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan2_single(ScanJob j0, ScanJob j1)

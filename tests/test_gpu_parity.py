@@ -75,8 +75,8 @@ def test_trajectory_10_steps(eng):
 
 
 def test_heavy_cells_global_path(eng):
-    """A blob dense enough that staged strips overflow LDS and take the global
-    path, mixed with sparse strips on the LDS path."""
+    """Two dense blobs (staged in several windows) in a sparse world whose
+    workgroups span more grid rows than the staged layout holds (global path)."""
     from particle_life import worlds
 
     print("\n[parity] case=heavy_blob_2d", flush=True)
@@ -95,6 +95,95 @@ def test_clustered_3d(eng):
                                    sigma=0.03, dim=3)
     eng.load_world(w)
     check_one_step(eng, w)
+
+
+DENSE = {
+    # name: (seed, n, types, rmax, dt, beta, blobs, sigma, dim) -- max cell occupancy
+    "blob_3d_4k_per_cell": (120, 30_000, 12, 0.02, 0.01, 0.25, 4, 0.006, 3),    # 4284
+    "blob_2d_7k_per_cell": (121, 30_000, 6, 0.02, 0.01, 0.3, 3, 0.006, 2),      # 7364
+    "blob_3d_c3_density": (122, 120_000, 12, 0.01, 0.01, 0.25, 8, 0.008, 3),    # 1392
+}
+
+
+@pytest.mark.parametrize("name", list(DENSE))
+def test_dense_neighbourhoods_windowed(eng, name):
+    """Neighbourhoods of thousands of particles per cell: the staged
+    neighbourhood exceeds the LDS buffer and is scanned in several windows."""
+    from particle_life import worlds
+
+    seed, n, t, rmax, dt, beta, nb, sig, dim = DENSE[name]
+    print(f"\n[parity] case={name} n={n}", flush=True)
+    w = worlds.gaussian_blob_world(seed, n, t, rmax=rmax, dt=dt, beta=beta, nblobs=nb,
+                                   sigma=sig, dim=dim)
+    eng.load_world(w)
+    st = check_one_step(eng, w)
+    print(f"  {st}", flush=True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tiny_staging_buffer(eng, dim):
+    """A staging buffer of a few dozen particles forces every workgroup through
+    many windows; results must not change."""
+    print(f"\n[parity] case=tiny_staging_{dim}d", flush=True)
+    w = _world(123 + dim, 20_000, 6, 0.03 if dim == 2 else 0.07, 0.01, dim=dim)
+    eng.set_tuning(stage_slack=0.01)
+    try:
+        eng.load_world(w)
+        for _ in range(2):
+            check_one_step(eng, w)
+    finally:
+        eng.set_tuning()
+
+
+def test_launch_plans_bit_identical(eng):
+    """Every home visits its candidates in the same order whatever the
+    workgroup size, list capacity and staging windows, so states after several
+    steps are bit-identical across launch plans."""
+    from particle_life import worlds
+
+    print("\n[parity] case=plans_bit_identical", flush=True)
+    w = worlds.gaussian_blob_world(126, 30_000, 6, rmax=0.03, dt=0.01, beta=0.3, nblobs=6,
+                                   sigma=0.03, dim=2)
+    plans = [dict(), dict(block_homes=64, list_cap=8, stage_slack=0.05),
+             dict(block_homes=128, list_cap=64, stage_slack=4.0),
+             dict(block_homes=256, list_cap=16, stage_slack=0.3, direct_launch=True)]
+    ref = None
+    for p in plans:
+        eng.set_tuning(**p)
+        eng.load_world(w)
+        eng.step(4)
+        st = eng.state()
+        got = (st["pos"], st["vel"], st["ids"], eng.pair_count())
+        if ref is None:
+            ref = got
+            continue
+        for k in range(3):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=f"plan {p}")
+        assert got[3] == ref[3]
+    eng.set_tuning()
+
+
+def test_planner_detects_clustering(eng):
+    """The automatic planner switches a clustered world to the clustered launch
+    shape once a step's clustering measure has reached the host, and a uniform
+    world back; results stay within tolerance of the oracle throughout."""
+    from particle_life import worlds
+
+    print("\n[parity] case=planner_regimes", flush=True)
+    eng.set_tuning()
+    w = worlds.gaussian_blob_world(122, 120_000, 12, rmax=0.01, dt=0.01, beta=0.25, nblobs=8,
+                                   sigma=0.008, dim=3)
+    eng.load_world(w)
+    eng.step(1)
+    eng.sync()
+    check_one_step(eng, w, debug=False)          # planned from the measured clustering
+    assert eng.tuning()["block_homes"] == 128
+    u = worlds.uniform_world(127, 50_000, 6, rmax=0.02, dt=0.01)
+    eng.load_world(u)
+    eng.step(1)
+    eng.sync()
+    check_one_step(eng, u, debug=False)
+    assert eng.tuning()["block_homes"] == 256
 
 
 def test_edge_sizes(eng):
