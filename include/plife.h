@@ -54,8 +54,8 @@ typedef struct {
 typedef struct {
     double step_ms;     /* wall time per step, event-to-event over all steps */
     double bin_ms;      /* cell keys + cell histogram + cell scan */
-    double sort_ms;     /* stable radix sort passes, SoA gather fused into the last */
-    double gather_ms;   /* separate SoA reorder (0 when fused into the sort) */
+    double sort_ms;     /* stable radix sort passes */
+    double gather_ms;   /* 0: the force kernel reads the pre-sort state through the permutation */
     double force_ms;    /* force + integrate kernel (average launch duration) */
     int64_t pairs;      /* interacting ordered pairs in the last step */
     int32_t steps;
@@ -151,6 +151,51 @@ int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out);
 /* Runs nsteps steps and returns the device time between two events on the
  * handle's stream bracketing them (ms). */
 int plife_time_steps(plife_physics *h, int32_t nsteps, double *ms);
+
+/* ------------------------------------------------------------------------
+ * Batches of independent worlds.  No counterpart in the reference, whose app
+ * steps one world per Physics object (Physics.java:112-134): a batch steps
+ * many worlds, each with its own settings, matrix and particle count, in the
+ * same launches.  Each world's step is the single-world step -- its results
+ * are bit-identical to stepping it alone in a plife_physics handle, whatever
+ * else the batch holds.  All worlds of a batch share dim.  Worlds join and
+ * leave between updates (continuous batching); ids are stable.  Limits:
+ * 65535 worlds, 2^31 - 1 particles, 2^28 cells in total.
+ * ------------------------------------------------------------------------ */
+typedef struct plife_batch plife_batch;
+
+int plife_batch_create(int32_t device, plife_batch **out);
+int plife_batch_destroy(plife_batch *b);
+
+/* Adds a world (settings, matrix and particles as for plife_set_settings and
+ * plife_set_particles; copied in) and returns its id.  Ids of its particles
+ * are 0..n-1 in the given order. */
+int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
+                          const double *matrix, int32_t n, const double *pos, const double *vel,
+                          const int32_t *types, int32_t *world_id);
+/* Removes a world; its slots are reclaimed at the next update. */
+int plife_batch_remove_world(plife_batch *b, int32_t world_id);
+/* New settings / matrix for one world (dim fixed; the matrix may grow). */
+int plife_batch_set_world_settings(plife_batch *b, int32_t world_id, const plife_settings *s,
+                                   int32_t ntypes, const double *matrix);
+
+/* nsteps steps of every world (enqueued; see plife_update). */
+int plife_batch_update(plife_batch *b, int32_t nsteps);
+int plife_batch_force_update_stop(plife_batch *b);
+int plife_batch_synchronize(plife_batch *b);
+
+/* One world's state in its current (cell-sorted) order; see plife_get_particles. */
+int plife_batch_get_world(plife_batch *b, int32_t world_id, int32_t n, double *pos, double *vel,
+                          int32_t *types, int32_t *ids);
+/* Particle count, interacting pairs of the last step and steps taken. */
+int plife_batch_world_info(plife_batch *b, int32_t world_id, int32_t *n, int64_t *pairs,
+                           int64_t *steps);
+int plife_batch_count(plife_batch *b, int32_t *nworlds, int64_t *nparticles);
+
+int plife_batch_set_tuning(plife_batch *b, const plife_tuning *t);
+int plife_batch_get_tuning(plife_batch *b, plife_tuning *t, int32_t *lds_bytes);
+int plife_batch_time_steps(plife_batch *b, int32_t nsteps, double *ms);
+int plife_batch_profile(plife_batch *b, int32_t nsteps, plife_timing *out);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,20 @@
 // plife.hip -- C ABI (include/plife.h) of the gfx950 particle-life physics step.
 //
-// Host orchestration of one Physics.update() (backend/Physics.java:112-134):
-//   bin (keys + cell histogram + scan) -> stable radix sort -> gather -> force+integrate,
+// Host orchestration of Physics.update() (backend/Physics.java:112-134) for a
+// batch of independent worlds; the single-world handle is a batch of one:
+//   bin (cell keys + cell histogram + first digit histogram) -> scans ->
+//   stable radix sort -> force + integrate (reads the pre-sort state through
+//   the permutation, writes the cell-sorted new state),
 // all enqueued on the handle's own stream.  No torch, no CPU fallback: every
 // step runs the HIP kernels of plife_kernels.hpp and plife_force.hpp.
+//
+// Batch layout: world w owns particle slots [off_w, off_w + n_w) and cells
+// [cell_base_w, cell_base_w + nx_w^dim).  Keys of different worlds never
+// interleave, so one stable sort of the batch keys leaves every world in its
+// own slot range, sorted exactly as stepping it alone sorts it; force
+// workgroups never straddle two worlds and every home visits its candidates
+// in the same order, so each world's results are bit-identical to stepping it
+// alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/plife.h"
@@ -52,6 +64,7 @@ int fail(int code, const char *fmt, ...)
 constexpr int kMaxTypes = 256;        // the app caps the matrix size at 256 (app/Main.java:745)
 constexpr int kMaxTypesLds = 32;      // matrix staged in LDS up to 32x32 (8 KB of FP64)
 constexpr long long kMaxCells = 1LL << 28;
+constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
 
@@ -81,20 +94,82 @@ void dev_free(T *&p)
     p = nullptr;
 }
 
+// Grow-only device array: reallocated (contents dropped) when too small.
+template <typename T>
+int ensure_dev(T **p, size_t *capv, size_t need)
+{
+    need = std::max<size_t>(need, 1);
+    if (need <= *capv && *p) return PLIFE_OK;
+    dev_free(*p);
+    *capv = 0;
+    int rc = dev_alloc(p, need);
+    if (rc) return rc;
+    *capv = need;
+    return PLIFE_OK;
+}
+
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+struct Grid {
+    int nx;
+    long long ncells;
+};
+
+Grid grid_of(const plife_settings &s)
+{
+    Grid g;
+    g.nx = (int)std::floor(1.0 / s.rmax);     // Physics.java:82-85, 312-313
+    g.ncells = s.dim == 3 ? (long long)g.nx * g.nx * g.nx : (long long)g.nx * g.nx;
+    return g;
+}
+
 }  // namespace
+
+// One world as the host keeps it.
+struct WorldHost {
+    int32_t id = 0;
+    plife_settings st{};
+    int ntypes = 0;
+    std::vector<double> matrix;
+    int n = 0, off = 0;
+    int64_t steps = 0;
+    bool live = true;                  // false once removed (dropped at the next layout)
+};
 
 struct plife_physics {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool batch = false;                // created by plife_batch_create
+    bool have_settings = false;        // single-world handle: plife_set_settings was called
+    int dim = 0;                       // shared by all worlds
 
-    // settings
-    bool have_settings = false;
-    plife_settings st{};
-    int ntypes = 0;
-    std::vector<double> matrix_host;
+    // worlds and their device layout
+    std::vector<WorldHost> worlds;
+    std::unordered_map<int32_t, int> index_of;   // world id -> index in worlds
+    int32_t next_id = 0;
+    int used = 0;                      // slots of the current buffers holding particles
+    bool layout_dirty = true;          // composition or particle counts changed
+    bool params_dirty = true;          // settings or matrices changed
+    uint64_t version = 0;              // bumped whenever the uploaded layout changes
+    std::vector<WorldDev> wdev;        // host copy of d_worlds
+    WorldDev *d_worlds = nullptr;
+    size_t worlds_cap = 0;
+    uint16_t *world_of = nullptr;      // world index of each slot (batches of several worlds)
+    size_t world_of_cap = 0;
+    int *d_src_off = nullptr;          // compaction source offsets
+    size_t src_off_cap = 0;
+    std::vector<double> matrices;      // all worlds' matrices, concatenated
     double *d_matrix = nullptr;
+    size_t matrix_cap = 0;
+    long long ncells = 0;              // all worlds
+    int ntypes_max = 0;
+
+    // force workgroups: (world, first home) per block; periodic worlds first
+    std::vector<uint2> blocks;
+    uint2 *d_blocks = nullptr;
+    size_t blocks_cap = 0;
+    int blocks_bs = 0, nb_wrap = 0, nb_total = 0;
+    uint64_t blocks_version = ~0ull;
 
     // particle state: positions ping-pong P[2], the rest A[2]
     int n = 0, cap = 0;
@@ -105,43 +180,47 @@ struct plife_physics {
 
     // binning / sort scratch
     uint32_t *keys0 = nullptr, *kA = nullptr, *kB = nullptr, *vA = nullptr, *vB = nullptr;
-    uint32_t *skeys = nullptr;         // cell key of each sorted slot (last radix pass)
+    uint32_t *skeys = nullptr;         // batch cell key of each sorted slot (last radix pass)
     uint32_t *perm = nullptr;          // points at vA or vB after a step
-    long long cells_cap = 0;
+    size_t cells_cap = 0;
     uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_end = nullptr;
     size_t hist_cap = 0, bsum_cap = 0;
     uint32_t *hist = nullptr, *bsum = nullptr;
-    unsigned long long *d_pairs = nullptr;   // [0] interacting pairs, [1] sum of cell counts^2
-    uint64_t *stat_host = nullptr;           // pinned copy of d_pairs[1], written every step
-    double occupancy = 0.0;                  // particle-weighted cell occupancy seen by the planner
-    bool clustered = false;                  // the planner's regime (with hysteresis)
+    // [0] interacting pairs, [1] sum of cell counts^2, [2 + w] pairs of world w
+    unsigned long long *d_pairs = nullptr;
+    size_t pairs_cap = 0;
+    uint64_t *stat_host = nullptr;     // pinned copy of d_pairs[1], written every step
+    double occupancy = 0.0;            // particle-weighted cell occupancy seen by the planner
+    bool clustered = false;            // the planner's regime (with hysteresis)
 
-    // grid of the last step
+    // the last step
     bool stepped = false;
-    int last_nx = 0;
     long long last_ncells = 0;
 
     // debug
     int debug = 0;
-    int dbg_cap = 0;
+    size_t dbg_cap = 0, dbg_hcap = 0;
     int32_t *dbg_count = nullptr;
     unsigned long long *dbg_hash = nullptr;
 
     std::atomic<int> stop{0};
-    bool cells_dirty = false;      // cell_count may hold stale counts
+    bool cells_dirty = false;          // cell_count may hold stale counts
 
     // HIP graph of two consecutive steps (buffer roles repeat every 2 steps)
     hipGraphExec_t gexec = nullptr;
     std::vector<unsigned char> gkey;
 
-    // launch tuning (0 = automatic) and what the last force launch used
+    // launch tuning (0 = automatic), the current plan, what the last launch used
     plife_tuning tune{};
+    ForcePlan plan{};
     size_t last_lds = 0;
     int last_cap = 0, last_bs = 0, last_lcap = 0;
 
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
 };
+
+struct plife_batch : plife_physics {};
 
 namespace {
 
@@ -151,18 +230,24 @@ int set_device(plife_physics *h)
     return PLIFE_OK;
 }
 
-void free_state(plife_physics *h)
+void free_state_arrays(uint32_t *px[2], uint32_t *py[2], uint32_t *pz[2], float *vx[2],
+                       float *vy[2], float *vz[2], int32_t *type[2], int32_t *id[2])
 {
     for (int k = 0; k < 2; ++k) {
-        dev_free(h->px[k]);
-        dev_free(h->py[k]);
-        dev_free(h->pz[k]);
-        dev_free(h->vx[k]);
-        dev_free(h->vy[k]);
-        dev_free(h->vz[k]);
-        dev_free(h->type[k]);
-        dev_free(h->id[k]);
+        dev_free(px[k]);
+        dev_free(py[k]);
+        dev_free(pz[k]);
+        dev_free(vx[k]);
+        dev_free(vy[k]);
+        dev_free(vz[k]);
+        dev_free(type[k]);
+        dev_free(id[k]);
     }
+}
+
+void free_state(plife_physics *h)
+{
+    free_state_arrays(h->px, h->py, h->pz, h->vx, h->vy, h->vz, h->type, h->id);
     dev_free(h->keys0);
     dev_free(h->kA);
     dev_free(h->kB);
@@ -173,22 +258,62 @@ void free_state(plife_physics *h)
     h->cap = 0;
 }
 
-int ensure_capacity(plife_physics *h, int n)
+// Room for `need` particles.  With keep, the particles in the first h->used
+// slots of the current buffers move to the new buffers.
+int ensure_capacity(plife_physics *h, int need, bool keep)
 {
-    if (n <= h->cap && h->cap > 0) return PLIFE_OK;
-    free_state(h);
-    const int cap = std::max(n, 1);
-    int rc;
-    for (int k = 0; k < 2; ++k) {
-        if ((rc = dev_alloc(&h->px[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->py[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->pz[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->vx[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->vy[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->vz[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->type[k], cap))) return rc;
-        if ((rc = dev_alloc(&h->id[k], cap))) return rc;
+    if (need <= h->cap && h->cap > 0) return PLIFE_OK;
+    const int cap = std::max({need, 1, keep ? h->cap + h->cap / 2 : 0});
+    uint32_t *px[2] = {}, *py[2] = {}, *pz[2] = {};
+    float *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
+    int32_t *type[2] = {}, *id[2] = {};
+    int rc = PLIFE_OK;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        if (!rc) rc = dev_alloc(&px[k], cap);
+        if (!rc) rc = dev_alloc(&py[k], cap);
+        if (!rc) rc = dev_alloc(&pz[k], cap);
+        if (!rc) rc = dev_alloc(&vx[k], cap);
+        if (!rc) rc = dev_alloc(&vy[k], cap);
+        if (!rc) rc = dev_alloc(&vz[k], cap);
+        if (!rc) rc = dev_alloc(&type[k], cap);
+        if (!rc) rc = dev_alloc(&id[k], cap);
     }
+    if (rc) {
+        free_state_arrays(px, py, pz, vx, vy, vz, type, id);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (keep && h->used > 0) {
+        const size_t b = (size_t)h->used * 4;
+        const int p = h->pc, a = h->ac;
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        if (e == hipSuccess) e = hipMemcpyAsync(px[0], h->px[p], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(py[0], h->py[p], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pz[0], h->pz[p], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(vx[0], h->vx[a], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(vy[0], h->vy[a], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(vz[0], h->vz[a], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(type[0], h->type[a], b, k, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(id[0], h->id[a], b, k, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        free_state_arrays(px, py, pz, vx, vy, vz, type, id);
+        return fail(PLIFE_ERR_DEVICE, "state copy failed: %s", hipGetErrorString(e));
+    }
+    free_state(h);
+    for (int k = 0; k < 2; ++k) {
+        h->px[k] = px[k];
+        h->py[k] = py[k];
+        h->pz[k] = pz[k];
+        h->vx[k] = vx[k];
+        h->vy[k] = vy[k];
+        h->vz[k] = vz[k];
+        h->type[k] = type[k];
+        h->id[k] = id[k];
+    }
+    h->pc = 0;
+    h->ac = 0;
     if ((rc = dev_alloc(&h->keys0, cap))) return rc;
     if ((rc = dev_alloc(&h->kA, cap))) return rc;
     if ((rc = dev_alloc(&h->kB, cap))) return rc;
@@ -201,46 +326,195 @@ int ensure_capacity(plife_physics *h, int n)
 
 int ensure_cells(plife_physics *h, long long ncells)
 {
-    if (ncells <= h->cells_cap) return PLIFE_OK;
+    const size_t c = (size_t)std::max(ncells, 1LL);
+    if (c <= h->cells_cap && h->cell_count) return PLIFE_OK;
     dev_free(h->cell_count);
     dev_free(h->cell_start);
     dev_free(h->cell_end);
     h->cells_cap = 0;
     int rc;
-    if ((rc = dev_alloc(&h->cell_count, (size_t)ncells))) return rc;
-    if ((rc = dev_alloc(&h->cell_start, (size_t)ncells))) return rc;
-    if ((rc = dev_alloc(&h->cell_end, (size_t)ncells))) return rc;
+    if ((rc = dev_alloc(&h->cell_count, c))) return rc;
+    if ((rc = dev_alloc(&h->cell_start, c))) return rc;
+    if ((rc = dev_alloc(&h->cell_end, c))) return rc;
     // the step keeps cell_count all-zero between steps (the scan clears it)
-    HIP_TRY(hipMemsetAsync(h->cell_count, 0, (size_t)ncells * 4, h->stream));
+    HIP_TRY(hipMemsetAsync(h->cell_count, 0, c * 4, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->cells_cap = ncells;
-    return PLIFE_OK;
-}
-
-int ensure_u32(uint32_t **p, size_t *capv, size_t need)
-{
-    if (need <= *capv && *p) return PLIFE_OK;
-    dev_free(*p);
-    *capv = 0;
-    int rc = dev_alloc(p, need);
-    if (rc) return rc;
-    *capv = need;
+    h->cells_cap = c;
     return PLIFE_OK;
 }
 
 int ensure_debug(plife_physics *h)
 {
-    if (!h->debug || h->dbg_cap >= std::max(h->n, 1)) return PLIFE_OK;
-    dev_free(h->dbg_count);
-    dev_free(h->dbg_hash);
-    h->dbg_cap = 0;
+    if (!h->debug) return PLIFE_OK;
+    const size_t need = (size_t)std::max(h->n, 1);
     int rc;
-    const int cap = std::max(h->n, 1);
-    if ((rc = dev_alloc(&h->dbg_count, cap))) return rc;
-    if ((rc = dev_alloc(&h->dbg_hash, cap))) return rc;
-    h->dbg_cap = cap;
+    if ((rc = ensure_dev(&h->dbg_count, &h->dbg_cap, need))) return rc;
+    return ensure_dev(&h->dbg_hash, &h->dbg_hcap, need);
+}
+
+// ---------------------------------------------------------------------------
+// world layout
+// ---------------------------------------------------------------------------
+
+WorldDev world_dev(const WorldHost &w, uint32_t cell_base, int moff)
+{
+    const plife_settings &s = w.st;
+    const Grid gr = grid_of(s);
+    const double R = s.rmax * kFix;
+    WorldDev d{};
+    d.off = w.off;
+    d.n = w.n;
+    d.nx = gr.nx;
+    d.all_global = gr.nx < 3 ? 1 : 0;   // repeated stencil cells (Physics.java:377-395)
+    d.cell_base = cell_base;
+    d.ntypes = w.ntypes;
+    d.moff = moff;
+    d.wrap = s.wrap ? 1 : 0;
+    d.r2hi = (float)(R * R * (1.0 + 0x1p-20));
+    d.cs = s.rmax;
+    d.r2u = (s.rmax * s.rmax) * (kFix * kFix);         // Physics.java:432, exact 2^64 scaling
+    d.S = 1.0 / R;
+    d.R = R;
+    d.beta = s.beta;
+    d.one_plus_beta = 1.0 + s.beta;
+    d.inv_1mb = 1.0 / (1.0 - s.beta);
+    d.inv_beta = 1.0 / s.beta;
+    d.ff = std::pow(s.friction, 60 * s.dt);            // Physics.java:401
+    d.kdt = s.force * s.dt / kFix;
+    d.dt = s.dt;
+    return d;
+}
+
+// Rebuilds the world table (offsets, cell bases, matrix offsets) from
+// h->worlds and uploads it with the matrices.
+int upload_params(plife_physics *h)
+{
+    const int W = (int)h->worlds.size();
+    h->wdev.resize(W);
+    h->matrices.clear();
+    long long cells = 0;
+    int tmax = 0;
+    for (int w = 0; w < W; ++w) {
+        const WorldHost &wh = h->worlds[w];
+        h->wdev[w] = world_dev(wh, (uint32_t)cells, (int)h->matrices.size());
+        h->matrices.insert(h->matrices.end(), wh.matrix.begin(), wh.matrix.end());
+        cells += grid_of(wh.st).ncells;
+        tmax = std::max(tmax, wh.ntypes);
+    }
+    if (cells > kMaxCells)
+        return fail(PLIFE_ERR_ARGUMENT, "the worlds need %lld cells together (limit %lld)", cells, kMaxCells);
+    h->ncells = cells;
+    h->ntypes_max = tmax;
+    int rc;
+    if ((rc = ensure_dev(&h->d_worlds, &h->worlds_cap, (size_t)W))) return rc;
+    if ((rc = ensure_dev(&h->d_matrix, &h->matrix_cap, h->matrices.size()))) return rc;
+    if ((rc = ensure_dev(&h->d_pairs, &h->pairs_cap, (size_t)(2 + W)))) return rc;
+    if (W) HIP_TRY(hipMemcpyAsync(h->d_worlds, h->wdev.data(), W * sizeof(WorldDev),
+                                  hipMemcpyHostToDevice, h->stream));
+    if (!h->matrices.empty())
+        HIP_TRY(hipMemcpyAsync(h->d_matrix, h->matrices.data(), h->matrices.size() * sizeof(double),
+                               hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));     // the host copies may change after this
     return PLIFE_OK;
 }
+
+// Brings the device layout up to date: drops removed worlds (one compaction
+// pass into the other state buffers), assigns offsets, uploads the world
+// table and the per-slot world index.  Called outside graph capture.
+int sync_layout(plife_physics *h)
+{
+    if (!h->layout_dirty && !h->params_dirty) return PLIFE_OK;
+    int rc;
+    if (h->layout_dirty) {
+        // old offsets of the surviving worlds, then the new dense layout
+        std::vector<WorldHost> kept;
+        std::vector<int> src;
+        bool moved = false, reindexed = false;
+        int off = 0;
+        for (WorldHost &w : h->worlds) {
+            if (!w.live) {
+                reindexed = true;
+                continue;
+            }
+            src.push_back(w.off);
+            if (w.off != off) moved = true;
+            w.off = off;
+            off += w.n;
+            kept.push_back(std::move(w));
+        }
+        h->worlds = std::move(kept);
+        h->index_of.clear();
+        for (int w = 0; w < (int)h->worlds.size(); ++w) h->index_of[h->worlds[w].id] = w;
+        const int W = (int)h->worlds.size();
+        const int nnew = off;
+        if ((rc = upload_params(h))) return rc;
+        if (reindexed)      // per-world pair counters of the last step refer to old indices
+            HIP_TRY(hipMemsetAsync(h->d_pairs, 0, (size_t)(2 + W) * 8, h->stream));
+        if (W > 1 && nnew > 0) {
+            if ((rc = ensure_dev(&h->world_of, &h->world_of_cap, (size_t)nnew))) return rc;
+            for (int w = 0; w < W; ++w)
+                if (h->worlds[w].n > 0)
+                    k_fill_u16<<<ceil_div(h->worlds[w].n, kBlock), kBlock, 0, h->stream>>>(
+                        h->world_of + h->worlds[w].off, (uint16_t)w, h->worlds[w].n);
+            LAUNCH_CHECK();
+        }
+        if (moved && nnew > 0) {
+            if ((rc = ensure_dev(&h->d_src_off, &h->src_off_cap, (size_t)W))) return rc;
+            HIP_TRY(hipMemcpyAsync(h->d_src_off, src.data(), W * sizeof(int), hipMemcpyHostToDevice,
+                                   h->stream));
+            const int p = h->pc, a = h->ac;
+            k_compact<<<ceil_div(nnew, kBlock), kBlock, 0, h->stream>>>(
+                nnew, h->d_worlds, W > 1 ? h->world_of : nullptr, h->d_src_off, h->px[p], h->py[p],
+                h->pz[p], h->vx[a], h->vy[a], h->vz[a], h->type[a], h->id[a], h->px[1 - p],
+                h->py[1 - p], h->pz[1 - p], h->vx[1 - a], h->vy[1 - a], h->vz[1 - a], h->type[1 - a],
+                h->id[1 - a]);
+            LAUNCH_CHECK();
+            h->pc = 1 - p;
+            h->ac = 1 - a;
+        }
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->n = nnew;
+        h->used = nnew;
+        h->stepped = false;
+    } else if ((rc = upload_params(h))) {
+        return rc;
+    }
+    h->layout_dirty = false;
+    h->params_dirty = false;
+    ++h->version;
+    return PLIFE_OK;
+}
+
+// Force workgroup table for block size bs: periodic worlds' blocks, then clamped.
+int ensure_blocks(plife_physics *h, int bs)
+{
+    if (h->blocks_bs == bs && h->blocks_version == h->version) return PLIFE_OK;
+    h->blocks.clear();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int w = 0; w < (int)h->worlds.size(); ++w) {
+            const WorldHost &wh = h->worlds[w];
+            if ((wh.st.wrap != 0) != (pass == 0)) continue;
+            for (int g = 0; g < wh.n; g += bs)
+                h->blocks.push_back(make_uint2((uint32_t)w, (uint32_t)(wh.off + g)));
+        }
+        if (pass == 0) h->nb_wrap = (int)h->blocks.size();
+    }
+    h->nb_total = (int)h->blocks.size();
+    int rc;
+    if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size()))) return rc;
+    if (!h->blocks.empty()) {
+        HIP_TRY(hipMemcpyAsync(h->d_blocks, h->blocks.data(), h->blocks.size() * sizeof(uint2),
+                               hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->blocks_bs = bs;
+    h->blocks_version = h->version;
+    return PLIFE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scans, sort, binning, force launches
+// ---------------------------------------------------------------------------
 
 struct RadixPlan {
     int bits, passes, D, ntiles;
@@ -260,7 +534,7 @@ RadixPlan radix_plan(long long ncells, int n)
     return p;
 }
 
-ScanJob no_job() { return ScanJob{nullptr, 0, nullptr, nullptr, 0, nullptr}; }
+ScanJob no_job() { return ScanJob{nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr}; }
 
 // One or two device-wide scans sharing three launches.
 int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
@@ -268,8 +542,8 @@ int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
     if (a.n <= 0) std::swap(a, b);
     if (a.n <= 0) return PLIFE_OK;
     const int nb0 = ceil_div(a.n, kScanTile), nb1 = b.n > 0 ? ceil_div(b.n, kScanTile) : 0;
-    int rc = ensure_u32(&h->bsum, &h->bsum_cap, (size_t)(nb0 + nb1));
-    if (rc) return rc;
+    if ((size_t)(nb0 + nb1) > h->bsum_cap || !h->bsum)
+        return fail(PLIFE_ERR_STATE, "scan scratch was not prepared");
     a.bsum = h->bsum;
     b.bsum = h->bsum + nb0;
     k_scan2_reduce<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
@@ -286,7 +560,7 @@ int radix_pass(plife_physics *h, const uint32_t *kin, const uint32_t *vin, uint3
     int rc;
     if (!hist_ready) {
         k_radix_hist<D><<<ntiles, kBlock, 0, h->stream>>>(kin, n, shift, ntiles, h->hist);
-        if ((rc = scan_jobs(h, ScanJob{h->hist, (1 << D) * ntiles, h->hist, nullptr, 0, nullptr},
+        if ((rc = scan_jobs(h, ScanJob{h->hist, (1 << D) * ntiles, h->hist, nullptr, 0, nullptr, nullptr},
                             no_job())))
             return rc;
     }
@@ -330,12 +604,14 @@ int radix_sort(plife_physics *h, int n, const RadixPlan &plan)
 }
 
 template <int DIM>
-void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, double cs, int nx, int ntiles)
+void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, int ntiles)
 {
-#define PLIFE_BIN(DD)                                                                         \
-    k_bin_tile<DIM, DD><<<ntiles, 1024, 0, h->stream>>>(P.x, P.y, P.z, n, cs, nx, ntiles,     \
-                                                          h->keys0, h->cell_count, h->hist,   \
-                                                          h->d_pairs)
+    const uint16_t *wof = h->worlds.size() > 1 ? h->world_of : nullptr;
+    const int ncounters = 2 + (int)h->worlds.size();
+#define PLIFE_BIN(DD)                                                                          \
+    k_bin_tile<DIM, DD><<<ntiles, 1024, 0, h->stream>>>(P.x, P.y, P.z, n, h->d_worlds, wof,    \
+                                                          ntiles, h->keys0, h->cell_count,     \
+                                                          h->hist, h->d_pairs, ncounters)
     switch (D) {
         case 1: PLIFE_BIN(1); break;
         case 2: PLIFE_BIN(2); break;
@@ -349,49 +625,42 @@ void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, double cs, int 
 #undef PLIFE_BIN
 }
 
-struct Grid {
-    int nx;
-    long long ncells;
-};
-
-Grid grid_of(const plife_settings &s)
-{
-    Grid g;
-    g.nx = (int)std::floor(1.0 / s.rmax);     // Physics.java:82-85, 312-313
-    g.ncells = s.dim == 3 ? (long long)g.nx * g.nx * g.nx : (long long)g.nx * g.nx;
-    return g;
-}
-
 template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
-void launch_force_bs(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
+void launch_force_bs(plife_physics *h, const ForceArgs &a, int nblocks, int bs, size_t lds)
 {
-    const long long grid = ((long long)a.n + bs - 1) / bs;
     switch (bs) {
-        case 64: k_force<DIM, WRAP, DEBUG, MLDS, 64><<<(int)grid, 64, lds, h->stream>>>(a); break;
-        case 128: k_force<DIM, WRAP, DEBUG, MLDS, 128><<<(int)grid, 128, lds, h->stream>>>(a); break;
-        default: k_force<DIM, WRAP, DEBUG, MLDS, 256><<<(int)grid, 256, lds, h->stream>>>(a); break;
+        case 64: k_force<DIM, WRAP, DEBUG, MLDS, 64><<<nblocks, 64, lds, h->stream>>>(a); break;
+        case 128: k_force<DIM, WRAP, DEBUG, MLDS, 128><<<nblocks, 128, lds, h->stream>>>(a); break;
+        default: k_force<DIM, WRAP, DEBUG, MLDS, 256><<<nblocks, 256, lds, h->stream>>>(a); break;
     }
 }
 
 template <int DIM, bool WRAP, bool DEBUG>
-void launch_force(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
+void launch_force(plife_physics *h, const ForceArgs &a, int nblocks, int bs, size_t lds)
 {
-    if (a.m_in_lds) launch_force_bs<DIM, WRAP, DEBUG, true>(h, a, bs, lds);
-    else launch_force_bs<DIM, WRAP, DEBUG, false>(h, a, bs, lds);
+    if (a.m_in_lds) launch_force_bs<DIM, WRAP, DEBUG, true>(h, a, nblocks, bs, lds);
+    else launch_force_bs<DIM, WRAP, DEBUG, false>(h, a, nblocks, bs, lds);
 }
 
-int launch_force_any(plife_physics *h, const ForceArgs &a, int bs, size_t lds)
+// One launch per wrap mode present in the batch (periodic worlds' blocks
+// come first in the block table).
+int launch_force_any(plife_physics *h, ForceArgs a, int bs, size_t lds)
 {
-    const int dim = h->st.dim;
-    const bool wrap = h->st.wrap != 0, dbg = h->debug != 0;
-    if (dim == 2) {
-        if (wrap) dbg ? launch_force<2, true, true>(h, a, bs, lds) : launch_force<2, true, false>(h, a, bs, lds);
-        else dbg ? launch_force<2, false, true>(h, a, bs, lds) : launch_force<2, false, false>(h, a, bs, lds);
-    } else {
-        if (wrap) dbg ? launch_force<3, true, true>(h, a, bs, lds) : launch_force<3, true, false>(h, a, bs, lds);
-        else dbg ? launch_force<3, false, true>(h, a, bs, lds) : launch_force<3, false, false>(h, a, bs, lds);
+    const bool dbg = h->debug != 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool wrap = pass == 0;
+        a.block_base = wrap ? 0 : h->nb_wrap;
+        const int nb = wrap ? h->nb_wrap : h->nb_total - h->nb_wrap;
+        if (nb <= 0) continue;
+        if (h->dim == 3) {
+            if (wrap) dbg ? launch_force<3, true, true>(h, a, nb, bs, lds) : launch_force<3, true, false>(h, a, nb, bs, lds);
+            else dbg ? launch_force<3, false, true>(h, a, nb, bs, lds) : launch_force<3, false, false>(h, a, nb, bs, lds);
+        } else {
+            if (wrap) dbg ? launch_force<2, true, true>(h, a, nb, bs, lds) : launch_force<2, true, false>(h, a, nb, bs, lds);
+            else dbg ? launch_force<2, false, true>(h, a, nb, bs, lds) : launch_force<2, false, false>(h, a, nb, bs, lds);
+        }
+        LAUNCH_CHECK();
     }
-    LAUNCH_CHECK();
     return PLIFE_OK;
 }
 
@@ -406,7 +675,11 @@ int record(plife_physics *h, PhaseEvents *ev, int k)
     return PLIFE_OK;
 }
 
-// Launch shape of the force kernel for the current world.  Explicit tuning
+// ---------------------------------------------------------------------------
+// launch planning and the step
+// ---------------------------------------------------------------------------
+
+// Launch shape of the force kernel for the current batch.  Explicit tuning
 // fields win.  Otherwise the planner reads the particle-weighted cell
 // occupancy (sum over cells of count^2, divided by N: the mean size of the
 // cell a particle sits in) measured by an earlier step:
@@ -417,10 +690,11 @@ int record(plife_physics *h, PhaseEvents *ev, int k)
 //     while the dense neighbourhoods stream through it in windows.
 // The plan never changes results: every home visits its candidates in the
 // same order whatever the windows and workgroup size.
-ForcePlan plan_force(const plife_physics *h, int dim, long long ncells, int n)
+ForcePlan plan_force(const plife_physics *h)
 {
+    const int dim = h->dim == 3 ? 3 : 2;
     const int NR = dim == 3 ? 9 : 3;
-    const double avg = (double)n / (double)std::max(ncells, 1LL);
+    const double avg = (double)h->n / (double)std::max(h->ncells, 1LL);
     const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
     const bool clustered = h->clustered;
     const plife_tuning &t = h->tune;
@@ -445,9 +719,10 @@ ForcePlan plan_force(const plife_physics *h, int dim, long long ncells, int n)
         budget = kClusteredLds;
     }
     want = std::min<long long>((want + 63) / 64 * 64, 65535);
-    const bool dbg = h->debug != 0, m_in_lds = h->ntypes <= kMaxTypesLds;
+    const bool dbg = h->debug != 0, m_in_lds = h->ntypes_max <= kMaxTypesLds;
     auto total = [&](long long c) {
-        return force_lds_layout(dim, dbg, h->ntypes, m_in_lds, (int)c, p.lcap, p.bs, p.max_cells).total;
+        return force_lds_layout(dim, dbg, h->ntypes_max, m_in_lds, (int)c, p.lcap, p.bs,
+                                p.max_cells).total;
     };
     // keep room for a useful staging window: shrink the cell table first
     const long long min_cap = std::min<long long>(want, 256);
@@ -455,7 +730,7 @@ ForcePlan plan_force(const plife_physics *h, int dim, long long ncells, int n)
     // then shrink the staging capacity until the layout fits the LDS budget
     while (want > 64 && total(want) > budget) want -= 64;
     p.cap = (int)std::max(0LL, want);
-    p.lds = force_lds_layout(dim, dbg, h->ntypes, m_in_lds, p.cap, p.lcap, p.bs, p.max_cells).total;
+    p.lds = total(p.cap);
     return p;
 }
 
@@ -464,9 +739,9 @@ ForcePlan plan_force(const plife_physics *h, int dim, long long ncells, int n)
 void refresh_occupancy(plife_physics *h)
 {
     const uint64_t q = __atomic_load_n(h->stat_host, __ATOMIC_RELAXED);
-    if (q == 0 || h->n <= 0) return;
+    if (q == 0 || h->n <= 0 || h->ncells <= 0) return;
     h->occupancy = (double)q / (double)h->n;
-    const double avg = (double)h->n / (double)std::max(grid_of(h->st).ncells, 1LL);
+    const double avg = (double)h->n / (double)h->ncells;
     // clustered: cells seen by a particle hold far more than the average (a
     // Poisson world has occupancy avg + 1) and enough to fill long windows
     const double ratio = h->occupancy / (avg + 1.0);
@@ -474,19 +749,36 @@ void refresh_occupancy(plife_physics *h)
     else if (h->clustered && (ratio < 3.0 || h->occupancy < 48.0)) h->clustered = false;
 }
 
+// Everything a step of the current configuration needs, so that the steps
+// (captured or direct) upload and allocate nothing.
+int prepare_step(plife_physics *h)
+{
+    int rc;
+    if ((rc = sync_layout(h))) return rc;
+    refresh_occupancy(h);
+    h->plan = plan_force(h);
+    if ((rc = ensure_cells(h, h->ncells))) return rc;
+    if ((rc = ensure_debug(h))) return rc;
+    const RadixPlan rp = radix_plan(h->ncells, h->n);
+    if ((rc = ensure_dev(&h->hist, &h->hist_cap, rp.hist_len))) return rc;
+    const size_t nb = (size_t)ceil_div(h->ncells, kScanTile) +
+                      (size_t)ceil_div((long long)rp.hist_len, kScanTile);
+    if ((rc = ensure_dev(&h->bsum, &h->bsum_cap, nb))) return rc;
+    return ensure_blocks(h, h->plan.bs);
+}
+
+// One step of every world.  prepare_step must have run since the last change.
 int do_step(plife_physics *h, PhaseEvents *ev)
 {
-    const plife_settings &s = h->st;
-    const Grid gr = grid_of(s);
     const int n = h->n;
+    const int nc = (int)h->ncells;
     int rc;
-    if ((rc = ensure_cells(h, gr.ncells))) return rc;
-    if ((rc = ensure_debug(h))) return rc;
-    const int nc = (int)gr.ncells;
+    if (h->blocks_bs != h->plan.bs || h->blocks_version != h->version || h->layout_dirty ||
+        h->params_dirty)
+        return fail(PLIFE_ERR_STATE, "the step was not prepared");
 
     if ((rc = record(h, ev, 0))) return rc;
-    const RadixPlan plan = radix_plan(gr.ncells, n);
-    if ((rc = ensure_u32(&h->hist, &h->hist_cap, plan.hist_len))) return rc;
+    const RadixPlan plan = radix_plan(h->ncells, n);
     // A step that failed between binning and scanning may leave counts behind.
     if (h->cells_dirty) {
         k_fill_u32<<<ceil_div(nc, kBlock), kBlock, 0, h->stream>>>(h->cell_count, 0u, nc);
@@ -498,15 +790,16 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     //    cleared for the next step) and the digit scan in one launch
     const PosBuf P0 = h->P(h->pc);
     if (n > 0) {
-        if (s.dim == 3) launch_bin<3>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
-        else launch_bin<2>(h, plan.D, P0, n, s.rmax, gr.nx, plan.ntiles);
+        if (h->dim == 3) launch_bin<3>(h, plan.D, P0, n, plan.ntiles);
+        else launch_bin<2>(h, plan.D, P0, n, plan.ntiles);
     } else {
-        k_zero_u64<<<1, 64, 0, h->stream>>>(h->d_pairs, 2);
+        const int nz = 2 + (int)h->worlds.size();
+        k_zero_u64<<<ceil_div(nz, kBlock), kBlock, 0, h->stream>>>(h->d_pairs, nz);
     }
     LAUNCH_CHECK();
     if ((rc = scan_jobs(h, ScanJob{h->cell_count, nc, h->cell_start, h->cell_end, 1, nullptr,
                                    h->d_pairs + 1},
-                        n > 0 ? ScanJob{h->hist, (int)plan.hist_len, h->hist, nullptr, 0, nullptr}
+                        n > 0 ? ScanJob{h->hist, (int)plan.hist_len, h->hist, nullptr, 0, nullptr, nullptr}
                               : no_job())))
         return rc;
     // the clustering measure travels to pinned host memory for the launch planner
@@ -516,8 +809,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     if ((rc = record(h, ev, 1))) return rc;
 
     h->stepped = true;
-    h->last_nx = gr.nx;
-    h->last_ncells = gr.ncells;
+    h->last_ncells = h->ncells;
     if (n == 0) {
         h->perm = h->vA;
         for (int k = 2; k < 5; ++k)
@@ -530,12 +822,11 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     if ((rc = record(h, ev, 2))) return rc;
     if ((rc = record(h, ev, 3))) return rc;
 
-    // (Physics.java:122-131 fused: the update is Jacobi)
-    const double R = s.rmax * kFix;
-    const double R2 = R * R;
-    // 3. force + integrate: reads the pre-sort state through the permutation
-    //    and writes the new state in cell order (the array swap, Physics.java:350-353)
+    // 3. force + integrate (Physics.java:122-131 fused: the update is Jacobi):
+    //    reads the pre-sort state through the permutation and writes the new
+    //    state in cell order (the array swap, Physics.java:350-353)
     const int pi = h->pc, ai = h->ac, po = 1 - h->pc, ao = 1 - h->ac;
+    const ForcePlan &fp = h->plan;
     ForceArgs a{};
     a.perm = h->perm;
     a.sx = h->px[pi];
@@ -554,86 +845,57 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.ovz = h->vz[ao];
     a.otype = h->type[ao];
     a.oid = h->id[ao];
+    a.skeys = h->skeys;
     a.cell_start = h->cell_start;
     a.cell_end = h->cell_end;
     a.matrix = h->d_matrix;
-    a.ntypes = h->ntypes;
-    a.n = n;
-    a.nx = gr.nx;
-    const ForcePlan fp = plan_force(h, s.dim, gr.ncells, n);
-    const bool m_in_lds = h->ntypes <= kMaxTypesLds;
-    a.m_in_lds = m_in_lds ? 1 : 0;
-    a.skeys = h->skeys;
+    a.worlds = h->d_worlds;
+    a.blocks = h->d_blocks;
+    a.ntypes_max = h->ntypes_max;
+    a.m_in_lds = h->ntypes_max <= kMaxTypesLds ? 1 : 0;
+    a.cap = fp.cap;
     a.lcap = fp.lcap;
     a.max_cells = fp.max_cells;
-    a.cap = fp.cap;
-    const int bs = fp.bs;
-    a.all_global = gr.nx < 3 ? 1 : 0;
-    a.r2hi = (float)(R2 * (1.0 + 0x1p-20));
-    a.r2u = (s.rmax * s.rmax) * (kFix * kFix);          // Physics.java:432, exact 2^64 scaling
-    a.S = 1.0 / R;
-    a.R = R;
-    a.beta = s.beta;
-    a.one_plus_beta = 1.0 + s.beta;
-    a.inv_1mb = 1.0 / (1.0 - s.beta);
-    a.inv_beta = 1.0 / s.beta;
-    a.ff = std::pow(s.friction, 60 * s.dt);             // Physics.java:401
-    a.kdt = s.force * s.dt / kFix;
-    a.cs = s.rmax;
-    a.dt = s.dt;
-    a.pair_counter = h->d_pairs;
+    a.pair_total = h->d_pairs;
+    a.pair_world = h->d_pairs + 2;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
-    const size_t lds = fp.lds;
-    h->last_lds = lds;
-    h->last_cap = a.cap;
-    h->last_bs = bs;
-    h->last_lcap = a.lcap;
-    if ((rc = launch_force_any(h, a, bs, lds))) return rc;
+    h->last_lds = fp.lds;
+    h->last_cap = fp.cap;
+    h->last_bs = fp.bs;
+    h->last_lcap = fp.lcap;
+    if ((rc = launch_force_any(h, a, fp.bs, fp.lds))) return rc;
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
     h->ac = ao;
     return PLIFE_OK;
 }
 
-// Allocate everything a step of the current configuration needs, so that a
-// captured step allocates nothing.
-int prepare_step(plife_physics *h)
-{
-    const Grid gr = grid_of(h->st);
-    int rc;
-    if ((rc = ensure_cells(h, gr.ncells))) return rc;
-    if ((rc = ensure_debug(h))) return rc;
-    const RadixPlan plan = radix_plan(gr.ncells, h->n);
-    if ((rc = ensure_u32(&h->hist, &h->hist_cap, plan.hist_len))) return rc;
-    const size_t nb = (size_t)ceil_div(gr.ncells, kScanTile) +
-                      (size_t)ceil_div((long long)plan.hist_len, kScanTile);
-    return ensure_u32(&h->bsum, &h->bsum_cap, nb);
-}
-
 std::vector<unsigned char> graph_key(const plife_physics *h)
 {
     struct Key {
-        int n, ntypes, debug, pc, ac;
-        plife_settings st;
+        int n, debug, pc, ac, dim, pad;
+        long long ncells;
+        uint64_t version;
         plife_tuning tune;
         ForcePlan plan;
-        const void *ptr[32];
+        const void *ptr[36];
     } k;
     std::memset(&k, 0, sizeof k);
     k.n = h->n;
-    k.ntypes = h->ntypes;
     k.debug = h->debug;
     k.pc = h->pc;
     k.ac = h->ac;
-    k.st = h->st;
+    k.dim = h->dim;
+    k.ncells = h->ncells;
+    k.version = h->version;
     k.tune = h->tune;
-    k.plan = plan_force(h, h->st.dim, grid_of(h->st).ncells, h->n);
+    k.plan = h->plan;
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
                        h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
                        h->cell_start, h->cell_end, h->hist, h->bsum, h->dbg_count, h->dbg_hash,
-                       h->d_matrix, h->d_pairs};
+                       h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks};
     std::memcpy(k.ptr, p, sizeof p);
     std::vector<unsigned char> v(sizeof k);
     std::memcpy(v.data(), &k, sizeof k);
@@ -647,7 +909,8 @@ void drop_graph(plife_physics *h)
     h->gkey.clear();
 }
 
-// Captures two consecutive steps into h->gexec (not executed yet).
+// Captures two consecutive steps into h->gexec (not executed yet; two steps
+// leave the buffer roles where they started).
 int capture_two_steps(plife_physics *h)
 {
     drop_graph(h);
@@ -670,13 +933,17 @@ int capture_two_steps(plife_physics *h)
     return PLIFE_OK;
 }
 
+void count_steps(plife_physics *h, int k)
+{
+    for (WorldHost &w : h->worlds) w.steps += k;
+}
+
 // nsteps steps: pairs of steps replay a captured graph, the rest launch directly.
 int run_steps(plife_physics *h, int nsteps, bool honour_stop)
 {
     int k = 0, rc;
-    refresh_occupancy(h);
+    if ((rc = prepare_step(h))) return rc;
     if (nsteps >= 2 && !h->tune.direct_launch) {
-        if ((rc = prepare_step(h))) return rc;
         std::vector<unsigned char> key = graph_key(h);
         if (!h->gexec || key != h->gkey) {
             if ((rc = capture_two_steps(h))) return rc;
@@ -686,20 +953,87 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
             if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
             HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
             k += 2;
+            count_steps(h, 2);
         }
     }
     for (; k < nsteps; ++k) {
         if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
         if ((rc = do_step(h, nullptr))) return rc;
+        count_steps(h, 1);
     }
     return PLIFE_OK;
 }
 
-int check_ready(plife_physics *h)
+int time_steps(plife_physics *h, int nsteps, double *ms)
 {
-    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
-    if (!h->have_settings) return fail(PLIFE_ERR_STATE, "plife_set_settings has not been called");
+    int rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    // build (or reuse) the step graph outside the timed region
+    if ((rc = prepare_step(h))) return rc;
+    if (nsteps >= 2 && !h->tune.direct_launch && (!h->gexec || graph_key(h) != h->gkey))
+        if ((rc = run_steps(h, 2, false))) return rc;
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    rc = run_steps(h, nsteps, false);
+    if (!rc) {
+        HIP_TRY(hipEventRecord(e1, h->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int pair_total(plife_physics *h, int64_t *pairs)
+{
+    *pairs = 0;
+    if (!h->d_pairs) return PLIFE_OK;
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, h->d_pairs, 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *pairs = (int64_t)v;
     return PLIFE_OK;
+}
+
+int profile(plife_physics *h, int nsteps, plife_timing *out)
+{
+    int rc;
+    if ((rc = prepare_step(h))) return rc;
+    std::vector<PhaseEvents> ev(nsteps);
+    for (auto &pe : ev)
+        for (auto &e : pe.e) HIP_TRY(hipEventCreate(&e));
+    for (int k = 0; k < nsteps && !rc; ++k) {
+        rc = do_step(h, &ev[k]);
+        if (!rc) count_steps(h, 1);
+    }
+    if (!rc) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        double sum[4] = {0, 0, 0, 0};
+        for (auto &pe : ev)
+            for (int p = 0; p < 4; ++p) {
+                float f = 0;
+                HIP_TRY(hipEventElapsedTime(&f, pe.e[p], pe.e[p + 1]));
+                sum[p] += f;
+            }
+        float wall = 0;
+        HIP_TRY(hipEventElapsedTime(&wall, ev.front().e[0], ev.back().e[4]));
+        out->bin_ms = sum[0] / nsteps;
+        out->sort_ms = sum[1] / nsteps;
+        out->gather_ms = sum[2] / nsteps;
+        out->force_ms = sum[3] / nsteps;
+        out->step_ms = wall / nsteps;
+        out->steps = nsteps;
+        int64_t pairs = 0;
+        rc = pair_total(h, &pairs);
+        out->pairs = pairs;
+    }
+    for (auto &pe : ev)
+        for (auto &e : pe.e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 uint32_t to_fixed(double x, bool wrap)
@@ -716,69 +1050,9 @@ uint32_t to_fixed(double x, bool wrap)
     return s >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(unsigned long long)s;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *plife_last_error(void) { return g_last_error.c_str(); }
-
-int plife_create(int32_t device, plife_physics **out)
+int check_settings(const plife_settings *s, int32_t ntypes, const double *matrix)
 {
-    if (!out) return fail(PLIFE_ERR_ARGUMENT, "out is null");
-    *out = nullptr;
-    int count = 0;
-    HIP_TRY(hipGetDeviceCount(&count));
-    if (device < 0 || device >= count)
-        return fail(PLIFE_ERR_ARGUMENT, "device %d out of range (%d devices)", device, count);
-    plife_physics *h = new plife_physics();
-    h->device = device;
-    int rc = set_device(h);
-    if (!rc) {
-        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) rc = fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-    }
-    if (!rc) rc = dev_alloc(&h->d_pairs, 2);
-    if (!rc) {
-        hipError_t e = hipHostMalloc((void **)&h->stat_host, sizeof(uint64_t), hipHostMallocDefault);
-        if (e != hipSuccess) rc = fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
-        else *h->stat_host = 0;
-    }
-    if (!rc) rc = dev_alloc(&h->d_matrix, kMaxTypes * kMaxTypes);
-    if (rc) {
-        plife_destroy(h);
-        return rc;
-    }
-    *out = h;
-    g_last_error.clear();
-    return PLIFE_OK;
-}
-
-int plife_destroy(plife_physics *h)
-{
-    if (!h) return PLIFE_OK;
-    (void)hipSetDevice(h->device);
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    drop_graph(h);
-    free_state(h);
-    dev_free(h->cell_count);
-    dev_free(h->cell_start);
-    dev_free(h->cell_end);
-    dev_free(h->hist);
-    dev_free(h->bsum);
-    dev_free(h->d_pairs);
-    if (h->stat_host) (void)hipHostFree(h->stat_host);
-    dev_free(h->d_matrix);
-    dev_free(h->dbg_count);
-    dev_free(h->dbg_hash);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
-    delete h;
-    return PLIFE_OK;
-}
-
-int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
-                       const double *matrix)
-{
-    if (!h || !s || !matrix) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (!s || !matrix) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     if (s->dim != 2 && s->dim != 3) return fail(PLIFE_ERR_ARGUMENT, "dim must be 2 or 3, got %d", s->dim);
     if (!(s->rmax > 0.0) || !(s->rmax <= 1.0))
         return fail(PLIFE_ERR_ARGUMENT, "rmax must be in (0, 1], got %g", s->rmax);
@@ -791,37 +1065,40 @@ int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes
     const Grid g = grid_of(*s);
     if (g.ncells > kMaxCells)
         return fail(PLIFE_ERR_ARGUMENT, "rmax %g gives %lld cells (limit %lld)", s->rmax, g.ncells, kMaxCells);
-    int rc = set_device(h);
-    if (rc) return rc;
-    const size_t mm = (size_t)ntypes * ntypes;
-    const bool changed = ntypes != h->ntypes || h->matrix_host.size() != mm ||
-                         std::memcmp(h->matrix_host.data(), matrix, mm * sizeof(double)) != 0;
-    if (changed) {
-        h->matrix_host.assign(matrix, matrix + mm);
-        HIP_TRY(hipMemcpyAsync(h->d_matrix, h->matrix_host.data(), mm * sizeof(double),
-                               hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-    }
-    h->ntypes = ntypes;
-    h->st = *s;
-    h->have_settings = true;
     return PLIFE_OK;
 }
 
-int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const double *vel,
-                        const int32_t *types)
+int check_particles(int32_t n, const double *pos, const double *vel, const int32_t *types, int ntypes)
 {
-    int rc = check_ready(h);
-    if (rc) return rc;
     if (n < 0) return fail(PLIFE_ERR_ARGUMENT, "n must be >= 0, got %d", n);
     if (n > 0 && (!pos || !vel || !types)) return fail(PLIFE_ERR_ARGUMENT, "null array");
     for (int i = 0; i < n; ++i)
-        if (types[i] < 0 || types[i] >= h->ntypes)
-            return fail(PLIFE_ERR_ARGUMENT, "particle %d has type %d outside [0, %d)", i, types[i], h->ntypes);
-    if ((rc = set_device(h))) return rc;
-    if ((rc = ensure_capacity(h, n))) return rc;
-    const bool wrap = h->st.wrap != 0;
-    const bool d3 = h->st.dim == 3;
+        if (types[i] < 0 || types[i] >= ntypes)
+            return fail(PLIFE_ERR_ARGUMENT, "particle %d has type %d outside [0, %d)", i, types[i], ntypes);
+    return PLIFE_OK;
+}
+
+// Stores new settings of a world; marks the device table stale only on change.
+void store_settings(plife_physics *h, WorldHost &w, const plife_settings *s, int32_t ntypes,
+                    const double *matrix)
+{
+    const size_t mm = (size_t)ntypes * ntypes;
+    const bool changed = std::memcmp(&w.st, s, sizeof *s) != 0 || ntypes != w.ntypes ||
+                         w.matrix.size() != mm ||
+                         std::memcmp(w.matrix.data(), matrix, mm * sizeof(double)) != 0;
+    if (!changed) return;
+    w.st = *s;
+    w.ntypes = ntypes;
+    w.matrix.assign(matrix, matrix + mm);
+    h->params_dirty = true;
+}
+
+// Converts n particles (Physics.ensurePosition, fixed point) and uploads them
+// to slots [off, off + n) of the current buffers, with ids 0..n-1.
+int upload_particles(plife_physics *h, int off, int n, const double *pos, const double *vel,
+                     const int32_t *types, bool wrap, bool d3)
+{
+    if (n == 0) return PLIFE_OK;
     std::vector<uint32_t> x(n), y(n), z(n);
     std::vector<float> fx(n), fy(n), fz(n);
     std::vector<int32_t> ids(n);
@@ -834,45 +1111,35 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
         fz[i] = d3 ? (float)vel[3 * i + 2] : 0.0f;
         ids[i] = i;
     }
-    h->pc = 0;
-    h->ac = 0;
-    h->n = n;
-    h->stepped = false;
-    if (n > 0) {
-        HIP_TRY(hipMemcpyAsync(h->px[0], x.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->py[0], y.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->pz[0], z.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->vx[0], fx.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->vy[0], fy.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->vz[0], fz.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->type[0], types, n * 4, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->id[0], ids.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-    }
+    const size_t b = (size_t)n * 4;
+    const int p = h->pc, a = h->ac;
+    const hipMemcpyKind k = hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(h->px[p] + off, x.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->py[p] + off, y.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->pz[p] + off, z.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->vx[a] + off, fx.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->vy[a] + off, fy.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->vz[a] + off, fz.data(), b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->type[a] + off, types, b, k, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->id[a] + off, ids.data(), b, k, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    // a new world: forget the clustering measure of the old one
-    *h->stat_host = 0;
-    h->occupancy = 0.0;
-    h->clustered = false;
     return PLIFE_OK;
 }
 
-int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
-                        int32_t *ids)
+// Downloads slots [off, off + n) of the current buffers.
+int download_particles(plife_physics *h, int off, int n, double *pos, double *vel, int32_t *types,
+                       int32_t *ids, bool d3)
 {
-    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
-    if (n != h->n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but the handle holds %d particles", n, h->n);
-    int rc = set_device(h);
-    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (n == 0) return PLIFE_OK;
     const PosBuf P = h->P(h->pc);
     const AuxBuf A = h->A(h->ac);
-    const bool d3 = h->have_settings && h->st.dim == 3;
+    const size_t b = (size_t)n * 4;
     if (pos) {
         std::vector<uint32_t> x(n), y(n), z(n);
-        HIP_TRY(hipMemcpy(x.data(), P.x, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(y.data(), P.y, n * 4, hipMemcpyDeviceToHost));
-        if (d3) HIP_TRY(hipMemcpy(z.data(), P.z, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(x.data(), P.x + off, b, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(y.data(), P.y + off, b, hipMemcpyDeviceToHost));
+        if (d3) HIP_TRY(hipMemcpy(z.data(), P.z + off, b, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             pos[3 * i] = (double)x[i] * kInvFix;
             pos[3 * i + 1] = (double)y[i] * kInvFix;
@@ -881,24 +1148,197 @@ int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, i
     }
     if (vel) {
         std::vector<float> x(n), y(n), z(n);
-        HIP_TRY(hipMemcpy(x.data(), A.vx, n * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(y.data(), A.vy, n * 4, hipMemcpyDeviceToHost));
-        if (d3) HIP_TRY(hipMemcpy(z.data(), A.vz, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(x.data(), A.vx + off, b, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(y.data(), A.vy + off, b, hipMemcpyDeviceToHost));
+        if (d3) HIP_TRY(hipMemcpy(z.data(), A.vz + off, b, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             vel[3 * i] = x[i];
             vel[3 * i + 1] = y[i];
             vel[3 * i + 2] = d3 ? (double)z[i] : 0.0;
         }
     }
-    if (types) HIP_TRY(hipMemcpy(types, A.type, n * 4, hipMemcpyDeviceToHost));
-    if (ids) HIP_TRY(hipMemcpy(ids, A.id, n * 4, hipMemcpyDeviceToHost));
+    if (types) HIP_TRY(hipMemcpy(types, A.type + off, b, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(hipMemcpy(ids, A.id + off, b, hipMemcpyDeviceToHost));
     return PLIFE_OK;
+}
+
+int create_handle(int32_t device, plife_physics *h)
+{
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(PLIFE_ERR_ARGUMENT, "device %d out of range (%d devices)", device, count);
+    h->device = device;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    e = hipHostMalloc((void **)&h->stat_host, sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
+    *h->stat_host = 0;
+    return PLIFE_OK;
+}
+
+void destroy_handle(plife_physics *h)
+{
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    free_state(h);
+    dev_free(h->cell_count);
+    dev_free(h->cell_start);
+    dev_free(h->cell_end);
+    dev_free(h->hist);
+    dev_free(h->bsum);
+    dev_free(h->d_pairs);
+    dev_free(h->d_matrix);
+    dev_free(h->d_worlds);
+    dev_free(h->world_of);
+    dev_free(h->d_src_off);
+    dev_free(h->d_blocks);
+    dev_free(h->dbg_count);
+    dev_free(h->dbg_hash);
+    if (h->stat_host) (void)hipHostFree(h->stat_host);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// A new world: forget the clustering measure of the old ones.
+void reset_planner(plife_physics *h)
+{
+    *h->stat_host = 0;
+    h->occupancy = 0.0;
+    h->clustered = false;
+}
+
+int check_single(plife_physics *h)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (h->batch) return fail(PLIFE_ERR_ARGUMENT, "batch handles take the plife_batch_* calls");
+    return PLIFE_OK;
+}
+
+int check_ready(plife_physics *h)
+{
+    int rc = check_single(h);
+    if (rc) return rc;
+    if (!h->have_settings) return fail(PLIFE_ERR_STATE, "plife_set_settings has not been called");
+    return PLIFE_OK;
+}
+
+int check_tuning(const plife_tuning *t)
+{
+    if (!t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
+        t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
+        (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
+         t->block_homes != 256))
+        return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
+    return PLIFE_OK;
+}
+
+void report_tuning(const plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
+{
+    *t = h->tune;
+    // automatic fields report what the last force launch used
+    if (t->list_cap == 0) t->list_cap = h->last_lcap;
+    if (t->block_homes == 0) t->block_homes = h->last_bs;
+    if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
+}
+
+WorldHost *find_world(plife_physics *h, int32_t world_id)
+{
+    auto it = h->index_of.find(world_id);
+    if (it == h->index_of.end()) return nullptr;
+    WorldHost &w = h->worlds[it->second];
+    return w.live ? &w : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *plife_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// single world
+// ---------------------------------------------------------------------------
+
+int plife_create(int32_t device, plife_physics **out)
+{
+    if (!out) return fail(PLIFE_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    plife_physics *h = new plife_physics();
+    int rc = create_handle(device, h);
+    if (rc) {
+        destroy_handle(h);
+        delete h;
+        return rc;
+    }
+    h->worlds.emplace_back();
+    h->index_of[0] = 0;
+    *out = h;
+    g_last_error.clear();
+    return PLIFE_OK;
+}
+
+int plife_destroy(plife_physics *h)
+{
+    if (!h) return PLIFE_OK;
+    if (h->batch) return fail(PLIFE_ERR_ARGUMENT, "batch handles take plife_batch_destroy");
+    destroy_handle(h);
+    delete h;
+    return PLIFE_OK;
+}
+
+int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
+                       const double *matrix)
+{
+    int rc = check_single(h);
+    if (rc) return rc;
+    if ((rc = check_settings(s, ntypes, matrix))) return rc;
+    store_settings(h, h->worlds[0], s, ntypes, matrix);
+    h->dim = s->dim;
+    h->have_settings = true;
+    return PLIFE_OK;
+}
+
+int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const double *vel,
+                        const int32_t *types)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    WorldHost &w = h->worlds[0];
+    if ((rc = check_particles(n, pos, vel, types, w.ntypes))) return rc;
+    if ((rc = set_device(h))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((rc = ensure_capacity(h, n, false))) return rc;
+    if ((rc = upload_particles(h, 0, n, pos, vel, types, w.st.wrap != 0, w.st.dim == 3))) return rc;
+    w.n = n;
+    w.off = 0;
+    h->used = n;
+    h->layout_dirty = true;
+    h->stepped = false;
+    reset_planner(h);
+    return PLIFE_OK;
+}
+
+int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
+                        int32_t *ids)
+{
+    int rc = check_single(h);
+    if (rc) return rc;
+    const WorldHost &w = h->worlds[0];
+    if (n != w.n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but the handle holds %d particles", n, w.n);
+    if ((rc = set_device(h))) return rc;
+    return download_particles(h, w.off, n, pos, vel, types, ids, h->have_settings && h->dim == 3);
 }
 
 int plife_get_count(plife_physics *h, int32_t *n)
 {
-    if (!h || !n) return fail(PLIFE_ERR_ARGUMENT, "null argument");
-    *n = h->n;
+    int rc = check_single(h);
+    if (rc) return rc;
+    if (!n) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    *n = h->worlds[0].n;
     return PLIFE_OK;
 }
 
@@ -932,7 +1372,7 @@ int plife_get_grid(plife_physics *h, int32_t *nx, int32_t *ncells)
 {
     int rc = check_ready(h);
     if (rc) return rc;
-    const Grid g = grid_of(h->st);
+    const Grid g = grid_of(h->worlds[0].st);
     if (nx) *nx = g.nx;
     if (ncells) *ncells = (int32_t)g.ncells;
     return PLIFE_OK;
@@ -941,12 +1381,12 @@ int plife_get_grid(plife_physics *h, int32_t *nx, int32_t *ncells)
 int plife_get_cells(plife_physics *h, int32_t *keys, int32_t *cell_end, int32_t *perm,
                     int32_t ncells)
 {
-    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    int rc = check_single(h);
+    if (rc) return rc;
     if (!h->stepped) return fail(PLIFE_ERR_STATE, "no step has run since the state was set");
     if (ncells != h->last_ncells)
         return fail(PLIFE_ERR_ARGUMENT, "ncells = %d but the last step used %lld cells", ncells, h->last_ncells);
-    int rc = set_device(h);
-    if (rc) return rc;
+    if ((rc = set_device(h))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (keys && h->n) HIP_TRY(hipMemcpy(keys, h->keys0, (size_t)h->n * 4, hipMemcpyDeviceToHost));
     if (cell_end) HIP_TRY(hipMemcpy(cell_end, h->cell_end, (size_t)ncells * 4, hipMemcpyDeviceToHost));
@@ -959,21 +1399,14 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs)
     if (!h || !pairs) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     int rc = set_device(h);
     if (rc) return rc;
-    unsigned long long v = 0;
-    HIP_TRY(hipMemcpyAsync(&v, h->d_pairs, 8, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    *pairs = (int64_t)v;
-    return PLIFE_OK;
+    return pair_total(h, pairs);
 }
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t)
 {
-    if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
-    if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
-        t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
-        (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
-         t->block_homes != 256))
-        return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    int rc = check_tuning(t);
+    if (rc) return rc;
     h->tune = *t;
     return PLIFE_OK;
 }
@@ -981,11 +1414,7 @@ int plife_set_tuning(plife_physics *h, const plife_tuning *t)
 int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
 {
     if (!h || !t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
-    *t = h->tune;
-    // automatic fields report what the last force launch used
-    if (t->list_cap == 0) t->list_cap = h->last_lcap;
-    if (t->block_homes == 0) t->block_homes = h->last_bs;
-    if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
+    report_tuning(h, t, lds_bytes);
     return PLIFE_OK;
 }
 
@@ -1016,28 +1445,7 @@ int plife_time_steps(plife_physics *h, int32_t nsteps, double *ms)
     if (rc) return rc;
     if (!ms || nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
     if ((rc = set_device(h))) return rc;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    // build (or reuse) the step graph outside the timed region
-    if (nsteps >= 2 && !h->tune.direct_launch) {
-        if ((rc = prepare_step(h))) return rc;
-        if (!h->gexec || graph_key(h) != h->gkey) {
-            if ((rc = run_steps(h, 2, false))) return rc;
-        }
-    }
-    HIP_TRY(hipEventRecord(e0, h->stream));
-    rc = run_steps(h, nsteps, false);
-    if (!rc) {
-        HIP_TRY(hipEventRecord(e1, h->stream));
-        HIP_TRY(hipEventSynchronize(e1));
-        float f = 0;
-        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
-        *ms = f;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return rc;
+    return time_steps(h, nsteps, ms);
 }
 
 int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out)
@@ -1046,34 +1454,188 @@ int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out)
     if (rc) return rc;
     if (!out || nsteps <= 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
     if ((rc = set_device(h))) return rc;
-    std::vector<PhaseEvents> ev(nsteps);
-    for (auto &pe : ev)
-        for (auto &e : pe.e) HIP_TRY(hipEventCreate(&e));
-    for (int k = 0; k < nsteps && !rc; ++k) rc = do_step(h, &ev[k]);
-    if (!rc) {
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        double sum[4] = {0, 0, 0, 0};
-        for (auto &pe : ev)
-            for (int p = 0; p < 4; ++p) {
-                float f = 0;
-                HIP_TRY(hipEventElapsedTime(&f, pe.e[p], pe.e[p + 1]));
-                sum[p] += f;
-            }
-        float wall = 0;
-        HIP_TRY(hipEventElapsedTime(&wall, ev.front().e[0], ev.back().e[4]));
-        out->bin_ms = sum[0] / nsteps;
-        out->sort_ms = sum[1] / nsteps;
-        out->gather_ms = sum[2] / nsteps;
-        out->force_ms = sum[3] / nsteps;
-        out->step_ms = wall / nsteps;
-        out->steps = nsteps;
-        int64_t pairs = 0;
-        rc = plife_get_pair_count(h, &pairs);
-        out->pairs = pairs;
+    return profile(h, nsteps, out);
+}
+
+// ---------------------------------------------------------------------------
+// batches of worlds
+// ---------------------------------------------------------------------------
+
+int plife_batch_create(int32_t device, plife_batch **out)
+{
+    if (!out) return fail(PLIFE_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    plife_batch *b = new plife_batch();
+    b->batch = true;
+    int rc = create_handle(device, b);
+    if (rc) {
+        destroy_handle(b);
+        delete b;
+        return rc;
     }
-    for (auto &pe : ev)
-        for (auto &e : pe.e) (void)hipEventDestroy(e);
-    return rc;
+    *out = b;
+    g_last_error.clear();
+    return PLIFE_OK;
+}
+
+int plife_batch_destroy(plife_batch *b)
+{
+    if (!b) return PLIFE_OK;
+    destroy_handle(b);
+    delete b;
+    return PLIFE_OK;
+}
+
+int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
+                          const double *matrix, int32_t n, const double *pos, const double *vel,
+                          const int32_t *types, int32_t *world_id)
+{
+    if (!b || !world_id) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    int rc = check_settings(s, ntypes, matrix);
+    if (rc) return rc;
+    if ((rc = check_particles(n, pos, vel, types, ntypes))) return rc;
+    int live = 0;
+    for (const WorldHost &w : b->worlds) live += w.live ? 1 : 0;
+    if (live > 0 && s->dim != b->dim)
+        return fail(PLIFE_ERR_ARGUMENT, "the batch holds %dD worlds, got dim %d", b->dim, s->dim);
+    if ((int)b->worlds.size() >= kMaxWorlds)
+        return fail(PLIFE_ERR_ARGUMENT, "a batch holds at most %d worlds", kMaxWorlds);
+    if ((long long)b->used + n > 0x7FFFFFFFLL)
+        return fail(PLIFE_ERR_ARGUMENT, "a batch holds at most 2^31 - 1 particles");
+    if ((rc = set_device(b))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((rc = ensure_capacity(b, b->used + n, true))) return rc;
+    if ((rc = upload_particles(b, b->used, n, pos, vel, types, s->wrap != 0, s->dim == 3))) return rc;
+    WorldHost w;
+    w.id = b->next_id++;
+    w.st = *s;
+    w.ntypes = ntypes;
+    w.matrix.assign(matrix, matrix + (size_t)ntypes * ntypes);
+    w.n = n;
+    w.off = b->used;
+    b->used += n;
+    b->dim = s->dim;
+    b->index_of[w.id] = (int)b->worlds.size();
+    b->worlds.push_back(std::move(w));
+    b->layout_dirty = true;
+    *world_id = b->worlds.back().id;
+    return PLIFE_OK;
+}
+
+int plife_batch_remove_world(plife_batch *b, int32_t world_id)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    WorldHost *w = find_world(b, world_id);
+    if (!w) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    w->live = false;
+    b->layout_dirty = true;
+    return PLIFE_OK;
+}
+
+int plife_batch_set_world_settings(plife_batch *b, int32_t world_id, const plife_settings *s,
+                                   int32_t ntypes, const double *matrix)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    int rc = check_settings(s, ntypes, matrix);
+    if (rc) return rc;
+    WorldHost *w = find_world(b, world_id);
+    if (!w) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    if (s->dim != b->dim) return fail(PLIFE_ERR_ARGUMENT, "the batch holds %dD worlds", b->dim);
+    if (s->wrap != w->st.wrap || ntypes < w->ntypes) {
+        // types already on the device must stay in range; wrap decides the workgroup order
+        if (ntypes < w->ntypes) return fail(PLIFE_ERR_ARGUMENT, "the matrix may not shrink below the world's types");
+        b->layout_dirty = true;
+    }
+    store_settings(b, *w, s, ntypes, matrix);
+    return PLIFE_OK;
+}
+
+int plife_batch_update(plife_batch *b, int32_t nsteps)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "nsteps must be >= 0");
+    int rc = set_device(b);
+    if (rc) return rc;
+    b->stop.store(0);
+    return run_steps(b, nsteps, true);
+}
+
+int plife_batch_force_update_stop(plife_batch *b) { return plife_force_update_stop(b); }
+
+int plife_batch_synchronize(plife_batch *b) { return plife_synchronize(b); }
+
+int plife_batch_get_world(plife_batch *b, int32_t world_id, int32_t n, double *pos, double *vel,
+                          int32_t *types, int32_t *ids)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    const WorldHost *w = find_world(b, world_id);
+    if (!w) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    if (n != w->n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but world %d holds %d particles", n, world_id, w->n);
+    int rc = set_device(b);
+    if (rc) return rc;
+    return download_particles(b, w->off, n, pos, vel, types, ids, b->dim == 3);
+}
+
+int plife_batch_world_info(plife_batch *b, int32_t world_id, int32_t *n, int64_t *pairs,
+                           int64_t *steps)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    const WorldHost *w = find_world(b, world_id);
+    if (!w) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    if (n) *n = w->n;
+    if (steps) *steps = w->steps;
+    if (pairs) {
+        *pairs = 0;
+        const int idx = b->index_of[world_id];
+        // counters are indexed by the layout of the last step
+        if (w->steps > 0 && !b->layout_dirty && b->d_pairs && 2 + idx < (int)b->pairs_cap) {
+            int rc = set_device(b);
+            if (rc) return rc;
+            unsigned long long v = 0;
+            HIP_TRY(hipMemcpyAsync(&v, b->d_pairs + 2 + idx, 8, hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            *pairs = (int64_t)v;
+        }
+    }
+    return PLIFE_OK;
+}
+
+int plife_batch_count(plife_batch *b, int32_t *nworlds, int64_t *nparticles)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    int32_t w = 0;
+    int64_t p = 0;
+    for (const WorldHost &wh : b->worlds)
+        if (wh.live) {
+            ++w;
+            p += wh.n;
+        }
+    if (nworlds) *nworlds = w;
+    if (nparticles) *nparticles = p;
+    return PLIFE_OK;
+}
+
+int plife_batch_set_tuning(plife_batch *b, const plife_tuning *t) { return plife_set_tuning(b, t); }
+
+int plife_batch_get_tuning(plife_batch *b, plife_tuning *t, int32_t *lds_bytes)
+{
+    return plife_get_tuning(b, t, lds_bytes);
+}
+
+int plife_batch_time_steps(plife_batch *b, int32_t nsteps, double *ms)
+{
+    if (!b || !ms || nsteps < 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
+    int rc = set_device(b);
+    if (rc) return rc;
+    return time_steps(b, nsteps, ms);
+}
+
+int plife_batch_profile(plife_batch *b, int32_t nsteps, plife_timing *out)
+{
+    if (!b || !out || nsteps <= 0) return fail(PLIFE_ERR_ARGUMENT, "bad argument");
+    int rc = set_device(b);
+    if (rc) return rc;
+    return profile(b, nsteps, out);
 }
 
 }  // extern "C"

@@ -37,20 +37,16 @@ struct ForceArgs {
     uint32_t *ox, *oy, *oz;
     float *ovx, *ovy, *ovz;
     int32_t *otype, *oid;
-    const uint32_t *skeys;           // cell key of each sorted slot
+    const uint32_t *skeys;           // batch cell key of each sorted slot
     const uint32_t *cell_start, *cell_end;
-    const double *matrix;            // ntypes*ntypes, row = acted-on type
-    int ntypes, m_in_lds;
-    int n, nx, cap, lcap, max_cells, all_global;
-    float r2hi;                      // screen bound, units 2^-64: (rmax 2^32)^2 (1 + 2^-20)
-    double r2u;                      // fl(rmax*rmax) * 2^64: the reference's bound in units 2^-64
-    double S;                        // 2^-32 / rmax: |rel| = |D| S with D in units 2^-32
-    double R;                        // rmax * 2^32:  1/|rel| = R / |D|
-    double beta, one_plus_beta, inv_1mb, inv_beta;
-    double ff;                       // friction^(60 dt) (host FP64 pow, Physics.java:401)
-    double kdt;                      // force * dt * 2^-32
-    double cs, dt;
-    unsigned long long *pair_counter;
+    const double *matrix;            // per world ntypes*ntypes at moff, row = acted-on type
+    const WorldDev *worlds;
+    const uint2 *blocks;             // per workgroup: (world, first home slot)
+    int block_base;                  // this launch's first entry of blocks
+    int ntypes_max, m_in_lds;
+    int cap, lcap, max_cells;
+    unsigned long long *pair_total;  // interacting pairs of the step, all worlds
+    unsigned long long *pair_world;  // per world
     int32_t *dbg_count;
     unsigned long long *dbg_hash;
 };
@@ -77,7 +73,7 @@ __device__ __forceinline__ unsigned long long id_hash(uint32_t id)
 // (app/Main.java:275-280).  acc accumulates D*(f/dist) in units of 2^-32.
 template <int DIM, bool DEBUG>
 __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, double m, uint32_t qid,
-                                       const ForceArgs &a)
+                                       const WorldDev &a)
 {
     const double dx = (double)dxi, dy = (double)dyi, dz = (double)dzi;
     double d2;
@@ -169,20 +165,20 @@ __device__ __forceinline__ uint32_t integrate_axis(uint32_t k, double v, double 
 }
 
 template <int DIM, bool WRAP, bool DEBUG>
-__device__ __forceinline__ void finish_home(const ForceArgs &a, const Acc &acc, uint32_t hx,
-                                            uint32_t hy, uint32_t hz, int htype, uint32_t o,
-                                            int g)
+__device__ __forceinline__ void finish_home(const ForceArgs &a, const WorldDev &w, const Acc &acc,
+                                            uint32_t hx, uint32_t hy, uint32_t hz, int htype,
+                                            uint32_t o, int g)
 {
-    const double nvx = a.ff * (double)a.vx[o] + a.kdt * acc.x;
-    const double nvy = a.ff * (double)a.vy[o] + a.kdt * acc.y;
+    const double nvx = w.ff * (double)a.vx[o] + w.kdt * acc.x;
+    const double nvy = w.ff * (double)a.vy[o] + w.kdt * acc.y;
     a.ovx[g] = (float)nvx;
     a.ovy[g] = (float)nvy;
-    a.ox[g] = integrate_axis(hx, nvx, a.dt, WRAP);
-    a.oy[g] = integrate_axis(hy, nvy, a.dt, WRAP);
+    a.ox[g] = integrate_axis(hx, nvx, w.dt, WRAP);
+    a.oy[g] = integrate_axis(hy, nvy, w.dt, WRAP);
     if constexpr (DIM == 3) {
-        const double nvz = a.ff * (double)a.vz[o] + a.kdt * acc.z;
+        const double nvz = w.ff * (double)a.vz[o] + w.kdt * acc.z;
         a.ovz[g] = (float)nvz;
-        a.oz[g] = integrate_axis(hz, nvz, a.dt, WRAP);
+        a.oz[g] = integrate_axis(hz, nvz, w.dt, WRAP);
     }
     a.otype[g] = htype;
     a.oid[g] = a.id[o];
@@ -197,13 +193,13 @@ __device__ __forceinline__ void finish_home(const ForceArgs &a, const Acc &acc, 
 // neighbourhood does not fit in LDS and for grids with fewer than 3 cells per
 // axis (where the reference visits repeated cells).
 template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
-__device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint32_t hx,
-                            uint32_t hy, uint32_t hz, int row, Acc &acc)
+__device__ void home_global(const ForceArgs &a, const WorldDev &w, const double *Mlds, int g,
+                            uint32_t hx, uint32_t hy, uint32_t hz, int row, Acc &acc)
 {
-    const int nx = a.nx;
-    const int cx0 = stencil_origin(hx, a.cs);
-    const int cy0 = stencil_origin(hy, a.cs);
-    const int cz0 = DIM == 3 ? stencil_origin(hz, a.cs) : 0;
+    const int nx = w.nx;
+    const int cx0 = stencil_origin(hx, w.cs);
+    const int cy0 = stencil_origin(hy, w.cs);
+    const int cz0 = DIM == 3 ? stencil_origin(hz, w.cs) : 0;
     const int zlo = DIM == 3 ? -1 : 0, zhi = DIM == 3 ? 1 : 0;
     for (int oz = zlo; oz <= zhi; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
@@ -218,7 +214,7 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
                            (DIM == 3 && (cz < 0 || cz >= nx))) {
                     continue;
                 }
-                const int ci = cx + cy * nx + cz * nx * nx;
+                const uint32_t ci = w.cell_base + (uint32_t)(cx + cy * nx + cz * nx * nx);
                 const int lo = (int)a.cell_start[ci], hi = (int)a.cell_end[ci];
                 for (int j = lo; j < hi; ++j) {
                     if (j == g) continue;
@@ -233,8 +229,8 @@ __device__ void home_global(const ForceArgs &a, const double *Mlds, int g, uint3
                         if (!ok) continue;
                     }
                     const int idx = row + a.type[jo];
-                    const double m = MLDS ? Mlds[idx] : a.matrix[idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, a);
+                    const double m = MLDS ? Mlds[idx] : a.matrix[w.moff + idx];
+                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, w);
                 }
             }
 }
@@ -284,7 +280,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     __shared__ uint32_t s_cnt[NW + 1];
     __shared__ unsigned long long s_pairs[NW];
 
-    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes, MLDS, a.cap, a.lcap, BS, a.max_cells);
+    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells);
     double *M = reinterpret_cast<double *>(smem + L.m);
     uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
     uint16_t *lists = reinterpret_cast<uint16_t *>(smem + L.lists);
@@ -295,29 +291,32 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     uint8_t *st = reinterpret_cast<uint8_t *>(smem + L.st);
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-    const int nx = a.nx;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int g0 = blk * BS;
-    const int nact = min(BS, a.n - g0);
+    const uint2 bw = a.blocks[a.block_base + blk];
+    const WorldDev w = a.worlds[bw.x];
+    const int nx = w.nx;
+    const uint32_t cb = w.cell_base;
+    const int g0 = (int)bw.y;
+    const int nact = min(BS, w.off + w.n - g0);
     const bool active = tid < nact;
     const int g = g0 + (active ? tid : nact - 1);
 
     if constexpr (MLDS)
-        for (int t = tid; t < a.ntypes * a.ntypes; t += BS) M[t] = a.matrix[t];
+        for (int t = tid; t < w.ntypes * w.ntypes; t += BS) M[t] = a.matrix[w.moff + t];
 
     // this thread's home: grid row (2D: cy; 3D: cy + cz*nx) and column
-    const uint32_t hkey = a.skeys[g];
+    const uint32_t hkey = a.skeys[g] - cb;
     const int hrow = (int)(hkey / (uint32_t)nx);
     const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
     const uint32_t ho = a.perm[g];
     const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
     const int htype = a.type[ho];
-    const int mrow = htype * a.ntypes;
+    const int mrow = htype * w.ntypes;
 
     // parts = distinct rows among the homes (rows are non-decreasing with tid)
-    const int prow = tid > 0 ? (int)(a.skeys[g - 1] / (uint32_t)nx) : -1;
+    const int prow = tid > 0 ? (int)((a.skeys[g - 1] - cb) / (uint32_t)nx) : -1;
     const bool pstart = active && (tid == 0 || prow != hrow);
-    const int nrow_next = (tid + 1 < nact) ? (int)(a.skeys[g + 1] / (uint32_t)nx) : -1;
+    const int nrow_next = (tid + 1 < nact) ? (int)((a.skeys[g + 1] - cb) / (uint32_t)nx) : -1;
     const bool pend = active && (tid + 1 == nact || nrow_next != hrow);
     const uint64_t wstarts = __ballot(pstart);
     if (lane == 0) s_cnt[wv] = (uint32_t)__popcll(wstarts);
@@ -348,7 +347,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         p_base[p + 1] = p_base[p] + NR * p_w[p];
     }
     const int NC = p_base[kMaxParts];
-    bool use_lds = !a.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
+    bool use_lds = !w.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
 
     // staged cells: start and count, then an exclusive prefix in LDS (carry ends
     // as the staged particle count, the same in every thread)
@@ -368,7 +367,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                 const int x = wrap_cell(pc - 1 + j, nx);
                 const int y = wrap_cell(cy + oy, nx);
                 const int z = DIM == 3 ? wrap_cell(cz + oz, nx) : 0;
-                const int ci = x + y * nx + z * nx * nx;
+                const uint32_t ci = cb + (uint32_t)(x + y * nx + z * nx * nx);
                 const uint32_t s0 = a.cell_start[ci];
                 c = a.cell_end[ci] - s0;
                 gs[f] = s0;
@@ -405,9 +404,9 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         // reference then scans columns {nx-1, 0, 1}: skip staged column nx-2
         // (column 1 is always farther than rmax).  Same for rows / layers.
         const int cy = hrow % nx, cz = DIM == 3 ? hrow / nx : 0;
-        const bool q2x = (hcx == nx - 1) && stencil_origin(hx, a.cs) == nx;
-        const bool q2y = (cy == nx - 1) && stencil_origin(hy, a.cs) == nx;
-        const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, a.cs) == nx;
+        const bool q2x = (hcx == nx - 1) && stencil_origin(hx, w.cs) == nx;
+        const bool q2y = (cy == nx - 1) && stencil_origin(hy, w.cs) == nx;
+        const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, w.cs) == nx;
         const int j_lo = jh - 1 + (q2x ? 1 : 0);
 
         // The staged neighbourhood is S particles; it is brought into LDS in
@@ -440,8 +439,8 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
                     const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
                     const int idx = mrow + (int)st[k];
-                    const double m = MLDS ? M[idx] : a.matrix[idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, a);
+                    const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
+                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
                 }
                 cnt = 0;
             };
@@ -463,7 +462,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                             const int k = klo + ii;
                             const uint2 q = sxy[k];
                             const uint32_t qz = DIM == 3 ? szz[k] : 0u;
-                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, a.r2hi)) {
+                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, w.r2hi)) {
                                 lists[cnt * BS + tid] = (uint16_t)k;
                                 ++cnt;
                             }
@@ -475,9 +474,9 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             __syncthreads();
         }
     } else if (active) {
-        home_global<DIM, WRAP, DEBUG, MLDS>(a, M, g, hx, hy, hz, mrow, acc);
+        home_global<DIM, WRAP, DEBUG, MLDS>(a, w, M, g, hx, hy, hz, mrow, acc);
     }
-    if (active) finish_home<DIM, WRAP, DEBUG>(a, acc, hx, hy, hz, htype, ho, g);
+    if (active) finish_home<DIM, WRAP, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, g);
 
     // interacting pairs, one 64-bit atomic per workgroup
     unsigned long long tot = active ? (unsigned long long)acc.cnt : 0ull;
@@ -489,7 +488,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         unsigned long long sum = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) sum += s_pairs[w];
-        if (sum) atomicAdd(a.pair_counter, sum);
+        if (sum) {
+            atomicAdd(a.pair_total, sum);
+            atomicAdd(a.pair_world + bw.x, sum);
+        }
     }
 }
 

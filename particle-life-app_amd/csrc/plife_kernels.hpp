@@ -97,6 +97,12 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_u16(uint16_t *p, uint16_t v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,6 +112,28 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 // ---------------------------------------------------------------------------
 // cell keys (Physics.java:362-375): cx = trunc(x / cs) in FP64, clamped to nx-1.
 // ---------------------------------------------------------------------------
+// One world of a batch (a single-world handle is a batch of one).  Worlds
+// occupy consecutive particle slots [off, off + n) in both the pre-sort and the
+// sorted order, and consecutive cells [cell_base, cell_base + nx^DIM) of the
+// batch's cell arrays, so one stable sort of the batch's cell keys sorts every
+// world exactly as stepping it alone would.
+struct WorldDev {
+    int off, n;                      // particle slots
+    int nx, all_global;              // cells per axis; 1 when nx < 3 (literal stencil)
+    uint32_t cell_base;              // first batch cell of this world
+    int ntypes, moff;                // matrix size and offset (doubles) in the batch matrix
+    int wrap;
+    float r2hi;                      // screen bound, units 2^-64: (rmax 2^32)^2 (1 + 2^-20)
+    double cs;                       // cell size = rmax (Physics.java:312)
+    double r2u;                      // fl(rmax*rmax) * 2^64: the reference's bound in units 2^-64
+    double S;                        // 2^-32 / rmax: |rel| = |D| S with D in units 2^-32
+    double R;                        // rmax * 2^32:  1/|rel| = R / |D|
+    double beta, one_plus_beta, inv_1mb, inv_beta;
+    double ff;                       // friction^(60 dt) (host FP64 pow, Physics.java:401)
+    double kdt;                      // force * dt * 2^-32
+    double dt;
+};
+
 __device__ __forceinline__ int key_axis(uint32_t k, double cs, int nx)
 {
     const double x = (double)k * kInvFix;   // exact
@@ -138,16 +166,24 @@ template <int DIM, int D>
 __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ px,
                                                   const uint32_t *__restrict__ py,
                                                   const uint32_t *__restrict__ pz, int n,
-                                                  double cs, int nx, int ntiles,
+                                                  const WorldDev *__restrict__ worlds,
+                                                  const uint16_t *__restrict__ world_of,
+                                                  int ntiles,
                                                   uint32_t *__restrict__ keys,
                                                   uint32_t *__restrict__ cell_count,
                                                   uint32_t *__restrict__ hist,
-                                                  unsigned long long *__restrict__ pairs)
+                                                  unsigned long long *__restrict__ counters,
+                                                  int ncounters)
 {
     constexpr int R = 1 << D;
     __shared__ uint32_t h[R];
     for (int t = threadIdx.x; t < R; t += 1024) h[t] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 2) pairs[threadIdx.x] = 0ull;   // pair count, sum of squares
+    // the step's counters (pairs, sum of squares, per-world pairs) start at zero
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < ncounters; t += 1024) counters[t] = 0ull;
+    const double cs0 = worlds[0].cs;
+    const int nx0 = worlds[0].nx;
+    const uint32_t base0 = worlds[0].cell_base;
     __syncthreads();
     const int base = blockIdx.x * kRadixTile;
 #pragma unroll
@@ -156,8 +192,18 @@ __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ 
         const bool valid = i < n;
         uint32_t key = 0xFFFFFFFFu;
         if (valid) {
+            double cs = cs0;
+            int nx = nx0;
+            uint32_t cb = base0;
+            if (world_of) {                            // several worlds: this slot's world
+                const WorldDev *w = worlds + world_of[i];
+                cs = w->cs;
+                nx = w->nx;
+                cb = w->cell_base;
+            }
             key = (uint32_t)key_axis(px[i], cs, nx) + (uint32_t)key_axis(py[i], cs, nx) * (uint32_t)nx;
             if constexpr (DIM == 3) key += (uint32_t)key_axis(pz[i], cs, nx) * (uint32_t)nx * (uint32_t)nx;
+            key += cb;
             keys[i] = key;
             atomicAdd(&h[key & (R - 1)], 1u);
         }
@@ -301,6 +347,39 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
 // ---------------------------------------------------------------------------
 // SoA state
 // ---------------------------------------------------------------------------
+// Compaction of a batch after worlds left: destination slot i of world w
+// (worlds[w].off <= i < worlds[w].off + worlds[w].n in the new layout) takes
+// the particle at src_off[w] + (i - worlds[w].off) of the old layout.
+__global__ __launch_bounds__(kBlock) void k_compact(int n, const WorldDev *__restrict__ worlds,
+                                                   const uint16_t *__restrict__ world_of,
+                                                   const int *__restrict__ src_off,
+                                                   const uint32_t *__restrict__ sx,
+                                                   const uint32_t *__restrict__ sy,
+                                                   const uint32_t *__restrict__ sz,
+                                                   const float *__restrict__ svx,
+                                                   const float *__restrict__ svy,
+                                                   const float *__restrict__ svz,
+                                                   const int32_t *__restrict__ stype,
+                                                   const int32_t *__restrict__ sid,
+                                                   uint32_t *__restrict__ dx, uint32_t *__restrict__ dy,
+                                                   uint32_t *__restrict__ dz, float *__restrict__ dvx,
+                                                   float *__restrict__ dvy, float *__restrict__ dvz,
+                                                   int32_t *__restrict__ dtype, int32_t *__restrict__ did)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int w = world_of ? (int)world_of[i] : 0;
+    const int j = src_off[w] + (i - worlds[w].off);
+    dx[i] = sx[j];
+    dy[i] = sy[j];
+    dz[i] = sz[j];
+    dvx[i] = svx[j];
+    dvy[i] = svy[j];
+    dvz[i] = svz[j];
+    dtype[i] = stype[j];
+    did[i] = sid[j];
+}
+
 struct PosBuf {
     uint32_t *x, *y, *z;
 };

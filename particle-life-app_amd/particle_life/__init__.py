@@ -1,8 +1,10 @@
 """particle_life -- MI355X (gfx950) physics step of particle-life.
 
 Host-side mirror of the reference's backend API (physics.py), the C-ABI
-binding (engine.py, _native.py) and seeded benchmark worlds (worlds.py).
+binding (engine.py, _native.py), batches of worlds (batch.py) and seeded
+benchmark worlds (worlds.py).
 """
+from .batch import Batch
 from .engine import Engine
 from .physics import (Accelerator, DefaultMatrix, DefaultMatrixGenerator, DefaultPositionSetter,
                       DefaultTypeSetter, GpuPhysics, Matrix, MatrixGenerator, Particle, Particles,
@@ -11,7 +13,7 @@ from .physics import (Accelerator, DefaultMatrix, DefaultMatrixGenerator, Defaul
 from . import worlds
 
 __all__ = [
-    "Accelerator", "DefaultMatrix", "DefaultMatrixGenerator", "DefaultPositionSetter",
+    "Accelerator", "Batch", "DefaultMatrix", "DefaultMatrixGenerator", "DefaultPositionSetter",
     "DefaultTypeSetter", "Engine", "GpuPhysics", "Matrix", "MatrixGenerator", "Particle",
     "Particles", "ParticleLifeAccelerator", "PhysicsSettings", "PhysicsSnapshot",
     "PositionSetter", "TypeSetter", "worlds",

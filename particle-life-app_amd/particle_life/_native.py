@@ -40,6 +40,21 @@ EXPORTS = (
     "plife_get_tuning",
     "plife_profile",
     "plife_time_steps",
+    "plife_batch_create",
+    "plife_batch_destroy",
+    "plife_batch_add_world",
+    "plife_batch_remove_world",
+    "plife_batch_set_world_settings",
+    "plife_batch_update",
+    "plife_batch_force_update_stop",
+    "plife_batch_synchronize",
+    "plife_batch_get_world",
+    "plife_batch_world_info",
+    "plife_batch_count",
+    "plife_batch_set_tuning",
+    "plife_batch_get_tuning",
+    "plife_batch_time_steps",
+    "plife_batch_profile",
 )
 
 
@@ -121,6 +136,22 @@ def load() -> ctypes.CDLL:
         "plife_get_tuning": [vp, ctypes.POINTER(PlifeTuning), ctypes.POINTER(i32)],
         "plife_profile": [vp, i32, ctypes.POINTER(PlifeTiming)],
         "plife_time_steps": [vp, i32, ctypes.POINTER(ctypes.c_double)],
+        "plife_batch_create": [i32, ctypes.POINTER(vp)],
+        "plife_batch_destroy": [vp],
+        "plife_batch_add_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32, vp, vp, vp,
+                                  ctypes.POINTER(i32)],
+        "plife_batch_remove_world": [vp, i32],
+        "plife_batch_set_world_settings": [vp, i32, ctypes.POINTER(PlifeSettings), i32, vp],
+        "plife_batch_update": [vp, i32],
+        "plife_batch_force_update_stop": [vp],
+        "plife_batch_synchronize": [vp],
+        "plife_batch_get_world": [vp, i32, i32, vp, vp, vp, vp],
+        "plife_batch_world_info": [vp, i32, ctypes.POINTER(i32), i64p, i64p],
+        "plife_batch_count": [vp, ctypes.POINTER(i32), i64p],
+        "plife_batch_set_tuning": [vp, ctypes.POINTER(PlifeTuning)],
+        "plife_batch_get_tuning": [vp, ctypes.POINTER(PlifeTuning), ctypes.POINTER(i32)],
+        "plife_batch_time_steps": [vp, i32, ctypes.POINTER(ctypes.c_double)],
+        "plife_batch_profile": [vp, i32, ctypes.POINTER(PlifeTiming)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -141,3 +141,32 @@ def config3(n: int = 4_000_000) -> World:
     [-1,1)^3, 0.015 in the unit box), rmax 0.02 (0.01), beta 0.25, dt 0.01 s."""
     return gaussian_blob_world(config_seed(3), n, 12, rmax=0.01, dt=0.01, beta=0.25,
                                name="c3_clustered_4m_3d")
+
+
+def config4_specs(nworlds: int = 512) -> list[dict]:
+    """Parameters of the batched workload: nworlds independent 2D worlds with
+    N log-uniform in [1000, 65536], types uniform in {3..12}, rmax uniform in
+    [0.01, 0.04] (unit box), beta uniform in [0.2, 0.4], half-life 0.04 s,
+    dt 0.01 s, periodic.  Drawn from the batch's own seed, so a prefix of the
+    list does not depend on nworlds."""
+    s = config_seed(4)
+    u_n, u_t, u_r, u_b = (uniform24(s, k, nworlds) for k in (101, 102, 103, 104))
+    lo, hi = np.log(1000.0), np.log(65536.0)
+    specs = []
+    for w in range(nworlds):
+        specs.append(dict(
+            seed=config_seed(4, w),
+            n=int(np.rint(np.exp(lo + (hi - lo) * u_n[w]))),
+            ntypes=3 + min(int(u_t[w] * 10), 9),
+            rmax=0.01 + 0.03 * float(u_r[w]),
+            beta=0.2 + 0.2 * float(u_b[w]),
+        ))
+    return specs
+
+
+def config4(nworlds: int = 512) -> list[World]:
+    """The batched workload's worlds (see config4_specs); about 7.9M particles
+    in total at 512 worlds."""
+    return [uniform_world(p["seed"], p["n"], p["ntypes"], rmax=p["rmax"], dt=0.01, beta=p["beta"],
+                          name=f"c4_world_{w}")
+            for w, p in enumerate(config4_specs(nworlds))]

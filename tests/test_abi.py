@@ -47,3 +47,16 @@ def test_null_arguments_are_rejected_without_a_device():
 def test_status_codes_map_to_java_exceptions():
     assert issubclass(_native.PlifeArgumentError, ValueError)
     assert issubclass(_native.PlifeError, RuntimeError)
+
+
+def test_batch_calls_reject_null_handles_without_a_device():
+    lib = _native.load()
+    s = _native.PlifeSettings(2, 1, 0.02, 0.85, 1.0, 0.02, 0.3)
+    wid = ctypes.c_int32()
+    assert lib.plife_batch_create(0, None) == _native.PLIFE_ERR_ARGUMENT
+    assert lib.plife_batch_add_world(None, ctypes.byref(s), 1, None, 0, None, None, None,
+                                     ctypes.byref(wid)) == _native.PLIFE_ERR_ARGUMENT
+    assert lib.plife_batch_update(None, 1) == _native.PLIFE_ERR_ARGUMENT
+    assert lib.plife_batch_remove_world(None, 0) == _native.PLIFE_ERR_ARGUMENT
+    assert lib.plife_batch_get_world(None, 0, 0, None, None, None, None) == _native.PLIFE_ERR_ARGUMENT
+    assert lib.plife_batch_destroy(None) == _native.PLIFE_OK
