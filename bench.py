@@ -40,9 +40,16 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--config", choices=("c1", "c2", "c3"), default="c2",
+    p.add_argument("--config", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
                    help="c2 (default): the 1M-particle 2D world the metric is quoted on; "
-                        "c1: the 10k 2D world; c3: the clustered 4M 3D world")
+                        "c1: the 10k 2D world; c3: the clustered 4M 3D world; "
+                        "c4: 512 independent 2D worlds stepped as one batch; "
+                        "c5: a Poisson stream of jobs served by continuous batching")
+    p.add_argument("--jobs", type=int, default=200, help="c5: jobs in the trace")
+    p.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
+    p.add_argument("--worlds", type=int, default=512, help="c4: number of worlds")
+    p.add_argument("--no-plain", action="store_true",
+                   help="c4/c5: skip the comparison with the plain (one world per handle) path")
     p.add_argument("--n", type=int, default=0, help="particle count override (0 = the config's)")
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-steps", type=int, default=3)
@@ -104,6 +111,132 @@ def cpu_baseline(world, steps: int, threads: int) -> dict:
     }
 
 
+def bench_batch(args) -> int:
+    """512 independent worlds (N log-uniform in [1000, 65536]) stepped as one
+    batch, against the plain path: one handle per world, each on its own
+    stream, stepped round-robin from the host."""
+    from particle_life import Batch, Engine, worlds
+
+    ws = worlds.config4(args.worlds)
+    ntot = sum(w.n for w in ws)
+    b = Batch(0)
+    ids = [b.add_world(w) for w in ws]
+    b.step(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    ms = b.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    pairs = sum(b.info(i)["pairs"] for i in ids)
+    prof = b.profile(args.profile_steps)
+    b.close()
+    steps_per_s = args.steps / (ms / 1e3)
+    out = {
+        "metric": f"pair-interactions/s & world-steps/s ({args.worlds}-world batch)",
+        "value": pairs * steps_per_s,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "steps_per_s": steps_per_s,
+        "world_steps_per_s": args.worlds * steps_per_s,
+        "particle_steps_per_s": ntot * steps_per_s,
+        "pairs_per_step": pairs,
+        "wall_s": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, f64 force law, u32 fixed-point positions",
+        "data": "synthetic (seeded SplitMix64 worlds)",
+        "config": {
+            "workload": f"{args.worlds} independent 2D worlds, N log-uniform in [1000, 65536] "
+                        f"(total {ntot}), 3-12 types, rmax U[0.01, 0.04] (unit box), "
+                        "beta U[0.2, 0.4], half-life 0.04 s, dt 0.01 s, periodic",
+            "n": ntot, "worlds": args.worlds,
+        },
+        "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
+    }
+    if not args.no_plain:
+        # the plain path: one handle (and stream) per world
+        engines = []
+        for w in ws:
+            e = Engine(0)
+            e.load_world(w)
+            engines.append(e)
+        k2 = max(2, args.steps // 2 * 2)
+        for e in engines:
+            e.step(2)
+        for e in engines:
+            e.sync()
+        t0 = time.perf_counter()
+        for _ in range(k2 // 2):
+            for e in engines:
+                e.step(2)
+        for e in engines:
+            e.sync()
+        plain_s = time.perf_counter() - t0
+        for e in engines:
+            e.close()
+        out["plain_path"] = {
+            "ms_per_step": plain_s * 1e3 / k2,
+            "world_steps_per_s": args.worlds * k2 / plain_s,
+            "steps": k2,
+            "how": "one handle per world (own stream, HIP graph of 2 steps), stepped round-robin",
+        }
+        out["speedup_vs_plain"] = out["world_steps_per_s"] / out["plain_path"]["world_steps_per_s"]
+    print(json.dumps(out))
+    return 0
+
+
+def bench_serve(args) -> int:
+    """A Poisson stream of jobs (2D and 3D worlds of 2k-262k particles run for
+    100-2000 steps) served by continuous batching, against the plain path
+    serving the same trace one job at a time."""
+    from particle_life import worlds
+    from particle_life.serve import serve, serve_alone
+
+    jobs = worlds.config5_jobs(args.jobs, rate=args.rate)
+    for j in jobs:
+        j["world"] = worlds.job_world(j)          # clients bring their initial state
+    served = serve(jobs)
+    s = served.summary()
+    out = {
+        "metric": "served job latency p99 (continuous batching)",
+        "value": s["p99_s"],
+        "unit": "s",
+        "n_gpus": 1,
+        "p50_s": s["p50_s"],
+        "p99_s": s["p99_s"],
+        "mean_s": s["mean_s"],
+        "jobs": s["jobs"],
+        "wall_s": s["wall_s"],
+        "job_steps_per_s": s["job_steps_per_s"],
+        "particle_steps_per_s": s["particle_steps_per_s"],
+        "updates": s["updates"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, f64 force law, u32 fixed-point positions",
+        "data": "synthetic (seeded SplitMix64 job trace)",
+        "config": {
+            "workload": f"{args.jobs} jobs, Poisson arrivals at {args.rate}/s; each a 2D or 3D "
+                        "periodic world, N log-uniform in [2000, 262144], 4-10 types, rmax "
+                        "U[0.01, 0.03] (unit box), beta 0.3, dt 0.01 s, 100-2000 steps; "
+                        "64 GB admission budget",
+            "jobs": args.jobs, "rate": args.rate,
+        },
+    }
+    if not args.no_plain:
+        alone = serve_alone(jobs).summary()
+        out["plain_path"] = {k: alone[k] for k in ("p50_s", "p99_s", "mean_s", "wall_s",
+                                                   "job_steps_per_s", "particle_steps_per_s")}
+        out["plain_path"]["how"] = "jobs one at a time in arrival order, each alone in one handle"
+        out["p99_speedup_vs_plain"] = alone["p99_s"] / s["p99_s"]
+        out["p50_speedup_vs_plain"] = alone["p50_s"] / s["p50_s"]
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.copy_calibration:
@@ -113,6 +246,10 @@ def main() -> int:
               "on a single MI355X and multi-GPU sharding is not part of this build.",
               file=sys.stderr)
         return 2
+    if args.config == "c4":
+        return bench_batch(args)
+    if args.config == "c5":
+        return bench_serve(args)
 
     from particle_life import Engine, worlds
 

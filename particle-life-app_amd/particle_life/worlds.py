@@ -170,3 +170,36 @@ def config4(nworlds: int = 512) -> list[World]:
     return [uniform_world(p["seed"], p["n"], p["ntypes"], rmax=p["rmax"], dt=0.01, beta=p["beta"],
                           name=f"c4_world_{w}")
             for w, p in enumerate(config4_specs(nworlds))]
+
+
+def config5_jobs(njobs: int = 200, rate: float = 40.0, max_steps: int = 2000) -> list[dict]:
+    """The served workload: a Poisson stream of jobs (rate jobs/s), each one
+    world with N log-uniform in [2000, 262144], dim 2 or 3 (even odds), 4-10
+    types, rmax uniform in [0.01, 0.03] (unit box), beta 0.3, half-life
+    0.04 s, dt 0.01 s, periodic, run for a step count uniform in
+    [100, max_steps].  Returns dicts with job, arrival (s), steps and the
+    world's parameters; job_world() builds the world itself."""
+    s = config_seed(5)
+    u = [uniform24(s, k, njobs) for k in (201, 202, 203, 204, 205, 206)]
+    lo, hi = np.log(2000.0), np.log(262144.0)
+    gaps = -np.log1p(-u[0]) / rate                   # exponential inter-arrival times
+    arrival = np.cumsum(gaps) - gaps[0]
+    jobs = []
+    for j in range(njobs):
+        jobs.append(dict(
+            job=j,
+            arrival=float(arrival[j]),
+            steps=100 + min(int(u[5][j] * (max_steps - 99)), max_steps - 100),
+            seed=config_seed(5, j),
+            n=int(np.rint(np.exp(lo + (hi - lo) * u[1][j]))),
+            dim=2 if u[2][j] < 0.5 else 3,
+            ntypes=4 + min(int(u[3][j] * 7), 6),
+            rmax=0.01 + 0.02 * float(u[4][j]),
+        ))
+    return jobs
+
+
+def job_world(job: dict) -> World:
+    """The world of one served job (config5_jobs)."""
+    return uniform_world(job["seed"], job["n"], job["ntypes"], rmax=job["rmax"], dt=0.01, beta=0.3,
+                         dim=job["dim"], name=f"c5_job_{job['job']}")

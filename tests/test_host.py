@@ -75,3 +75,34 @@ def test_gpu_physics_rejects_other_accelerators():
 
     with pytest.raises(NotImplementedError):
         physics.GpuPhysics(Custom())
+
+
+def test_served_trace_generator():
+    from particle_life import worlds
+    from particle_life.serve import footprint
+
+    js = worlds.config5_jobs(300, rate=40.0)
+    arr = np.array([j["arrival"] for j in js])
+    assert arr[0] == 0.0 and np.all(np.diff(arr) >= 0)
+    assert 3.0 < arr[-1] < 12.0                                  # ~300 / 40 s
+    n = np.array([j["n"] for j in js])
+    assert n.min() >= 2000 and n.max() <= 262144
+    assert 35_000 < n.mean() < 70_000                            # log-uniform mean ~53k
+    assert {j["dim"] for j in js} == {2, 3}
+    assert all(100 <= j["steps"] <= 2000 for j in js)
+    assert all(4 <= j["ntypes"] <= 10 and 0.01 <= j["rmax"] <= 0.03 for j in js)
+    w = worlds.job_world(js[0])
+    assert w.n == js[0]["n"] and w.dim == js[0]["dim"]
+    assert footprint(1000, 0.02, 2) == 1000 * 90 + 12 * 2500
+
+
+def test_batched_workload_generator():
+    from particle_life import worlds
+
+    specs = worlds.config4_specs(512)
+    assert specs[:10] == worlds.config4_specs(10)                # prefix-stable
+    n = np.array([p["n"] for p in specs])
+    assert n.min() >= 1000 and n.max() <= 65536
+    assert 7.0e6 < n.sum() < 8.8e6                               # ~7.9M expected
+    assert {p["ntypes"] for p in specs} == set(range(3, 13))
+    assert all(0.01 <= p["rmax"] <= 0.04 and 0.2 <= p["beta"] <= 0.4 for p in specs)
