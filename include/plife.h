@@ -89,6 +89,15 @@ int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, i
 
 int plife_get_count(plife_physics *h, int32_t *n);
 
+/* Seeded device-side world generation: n particles of the app's default
+ * presets -- uniform positions (backend/DefaultPositionSetter.java), uniform
+ * types in [0, ntypes) (backend/DefaultTypeSetter.java), velocity 0
+ * (Physics.java:297-303) -- drawn on the GPU from counter-based SplitMix64
+ * streams of `seed` (positions on the 2^-24 grid; ids 0..n-1), replacing the
+ * particles as plife_set_particles does, with no host transfer.  The
+ * reference draws from unseeded Math.random; this is its reproducible form. */
+int plife_generate_particles(plife_physics *h, int32_t n, uint64_t seed);
+
 /* Physics.update() (Physics.java:112-134) applied nsteps times.  Enqueues the
  * steps on the handle's stream and returns; plife_synchronize or a download
  * waits.  Returns PLIFE_STOPPED if plife_force_update_stop() arrived between
@@ -173,6 +182,11 @@ int plife_batch_destroy(plife_batch *b);
 int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
                           const double *matrix, int32_t n, const double *pos, const double *vel,
                           const int32_t *types, int32_t *world_id);
+/* Adds a world whose particles are generated on the device from `seed`, as
+ * plife_generate_particles does. */
+int plife_batch_add_generated_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
+                                    const double *matrix, int32_t n, uint64_t seed,
+                                    int32_t *world_id);
 /* Removes a world; its slots are reclaimed at the next update. */
 int plife_batch_remove_world(plife_batch *b, int32_t world_id);
 /* New settings / matrix for one world (dim fixed; the matrix may grow). */

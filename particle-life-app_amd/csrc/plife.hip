@@ -701,7 +701,7 @@ ForcePlan plan_force(const plife_physics *h)
     ForcePlan p;
     p.bs = t.block_homes ? t.block_homes : (clustered ? 128 : 256);
     p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))
-                            : (clustered ? 32 : occ >= 12.0 ? 24 : 16);
+                            : (clustered || occ >= 12.0 ? 32 : 16);
     p.lcap = std::min(p.lcap, 16384 / p.bs);             // lists stay within 32 KB of LDS
     // a workgroup of bs homes spans ~bs/avg + 1 cells of a row; it stages NR
     // rows of (span + 2) cells.  Clustered worlds keep a short cell table: the
@@ -1126,6 +1126,32 @@ int upload_particles(plife_physics *h, int off, int n, const double *pos, const 
     return PLIFE_OK;
 }
 
+// Generates n particles of the default presets from `seed` into slots
+// [off, off + n) of the current buffers (ids 0..n-1).
+int generate_particles(plife_physics *h, int off, int n, uint64_t seed, int ntypes, int dim)
+{
+    if (n == 0) return PLIFE_OK;
+    GenArgs g{};
+    for (int s = 0; s < 4; ++s) g.base[s] = splitmix_mix(seed + (uint64_t)(s + 1) * kStream);
+    g.n = n;
+    g.off = off;
+    g.ntypes = ntypes;
+    g.dim = dim;
+    const int p = h->pc, a = h->ac;
+    g.px = h->px[p];
+    g.py = h->py[p];
+    g.pz = h->pz[p];
+    g.vx = h->vx[a];
+    g.vy = h->vy[a];
+    g.vz = h->vz[a];
+    g.type = h->type[a];
+    g.id = h->id[a];
+    k_gen_uniform<<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(g);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PLIFE_OK;
+}
+
 // Downloads slots [off, off + n) of the current buffers.
 int download_particles(plife_physics *h, int off, int n, double *pos, double *vel, int32_t *types,
                        int32_t *ids, bool d3)
@@ -1322,6 +1348,25 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
     return PLIFE_OK;
 }
 
+int plife_generate_particles(plife_physics *h, int32_t n, uint64_t seed)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (n < 0) return fail(PLIFE_ERR_ARGUMENT, "n must be >= 0, got %d", n);
+    WorldHost &w = h->worlds[0];
+    if ((rc = set_device(h))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((rc = ensure_capacity(h, n, false))) return rc;
+    if ((rc = generate_particles(h, 0, n, seed, w.ntypes, w.st.dim))) return rc;
+    w.n = n;
+    w.off = 0;
+    h->used = n;
+    h->layout_dirty = true;
+    h->stepped = false;
+    reset_planner(h);
+    return PLIFE_OK;
+}
+
 int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
                         int32_t *ids)
 {
@@ -1506,6 +1551,42 @@ int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntype
     HIP_TRY(hipStreamSynchronize(b->stream));
     if ((rc = ensure_capacity(b, b->used + n, true))) return rc;
     if ((rc = upload_particles(b, b->used, n, pos, vel, types, s->wrap != 0, s->dim == 3))) return rc;
+    WorldHost w;
+    w.id = b->next_id++;
+    w.st = *s;
+    w.ntypes = ntypes;
+    w.matrix.assign(matrix, matrix + (size_t)ntypes * ntypes);
+    w.n = n;
+    w.off = b->used;
+    b->used += n;
+    b->dim = s->dim;
+    b->index_of[w.id] = (int)b->worlds.size();
+    b->worlds.push_back(std::move(w));
+    b->layout_dirty = true;
+    *world_id = b->worlds.back().id;
+    return PLIFE_OK;
+}
+
+int plife_batch_add_generated_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
+                                    const double *matrix, int32_t n, uint64_t seed,
+                                    int32_t *world_id)
+{
+    if (!b || !world_id) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    int rc = check_settings(s, ntypes, matrix);
+    if (rc) return rc;
+    if (n < 0) return fail(PLIFE_ERR_ARGUMENT, "n must be >= 0, got %d", n);
+    int live = 0;
+    for (const WorldHost &w : b->worlds) live += w.live ? 1 : 0;
+    if (live > 0 && s->dim != b->dim)
+        return fail(PLIFE_ERR_ARGUMENT, "the batch holds %dD worlds, got dim %d", b->dim, s->dim);
+    if ((int)b->worlds.size() >= kMaxWorlds)
+        return fail(PLIFE_ERR_ARGUMENT, "a batch holds at most %d worlds", kMaxWorlds);
+    if ((long long)b->used + n > 0x7FFFFFFFLL)
+        return fail(PLIFE_ERR_ARGUMENT, "a batch holds at most 2^31 - 1 particles");
+    if ((rc = set_device(b))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((rc = ensure_capacity(b, b->used + n, true))) return rc;
+    if ((rc = generate_particles(b, b->used, n, seed, ntypes, s->dim))) return rc;
     WorldHost w;
     w.id = b->next_id++;
     w.st = *s;

@@ -380,6 +380,48 @@ __global__ __launch_bounds__(kBlock) void k_compact(int n, const WorldDev *__res
     did[i] = sid[j];
 }
 
+// Seeded world generation, the device side of the default presets
+// (backend/DefaultPositionSetter.java: uniform positions; backend/
+// DefaultTypeSetter.java: uniform types; velocities 0, Physics.java:297-303).
+// Counter-based SplitMix64 (Vigna's published generator): draw k of stream s
+// of `seed` is mix(base_s + (k + 1) * gamma), base_s = mix(seed + s * kStream).
+// Uniforms are the top 24 bits (a 2^-24 grid, exact in FP32, FP64 and the
+// 32-bit fixed point); streams 1..3 are x, y, z and 4 the type.
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kStream = 0xD1B54A32D192ED03ull;
+
+__host__ __device__ inline uint64_t splitmix_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct GenArgs {
+    uint64_t base[4];                  // stream bases for x, y, z, type
+    int n, off, ntypes, dim;
+    uint32_t *px, *py, *pz;
+    float *vx, *vy, *vz;
+    int32_t *type, *id;
+};
+
+__global__ __launch_bounds__(kBlock) void k_gen_uniform(GenArgs g)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= g.n) return;
+    const uint64_t step = (uint64_t)(k + 1) * kGamma;
+    const int i = g.off + k;
+    g.px[i] = (uint32_t)(splitmix_mix(g.base[0] + step) >> 40) << 8;      // u24 * 2^-24 in 2^-32 units
+    g.py[i] = (uint32_t)(splitmix_mix(g.base[1] + step) >> 40) << 8;
+    g.pz[i] = g.dim == 3 ? (uint32_t)(splitmix_mix(g.base[2] + step) >> 40) << 8 : 0u;
+    const uint64_t m = splitmix_mix(g.base[3] + step) >> 40;            // floor(u24 * T) exactly
+    g.type[i] = (int32_t)((m * (uint64_t)g.ntypes) >> 24);
+    g.vx[i] = 0.0f;
+    g.vy[i] = 0.0f;
+    g.vz[i] = 0.0f;
+    g.id[i] = k;
+}
+
 struct PosBuf {
     uint32_t *x, *y, *z;
 };

@@ -27,6 +27,7 @@ EXPORTS = (
     "plife_set_particles",
     "plife_get_particles",
     "plife_get_count",
+    "plife_generate_particles",
     "plife_update",
     "plife_force_update_stop",
     "plife_synchronize",
@@ -43,6 +44,7 @@ EXPORTS = (
     "plife_batch_create",
     "plife_batch_destroy",
     "plife_batch_add_world",
+    "plife_batch_add_generated_world",
     "plife_batch_remove_world",
     "plife_batch_set_world_settings",
     "plife_batch_update",
@@ -124,6 +126,7 @@ def load() -> ctypes.CDLL:
         "plife_set_particles": [vp, i32, vp, vp, vp],
         "plife_get_particles": [vp, i32, vp, vp, vp, vp],
         "plife_get_count": [vp, ctypes.POINTER(i32)],
+        "plife_generate_particles": [vp, i32, ctypes.c_uint64],
         "plife_update": [vp, i32],
         "plife_force_update_stop": [vp],
         "plife_synchronize": [vp],
@@ -140,6 +143,8 @@ def load() -> ctypes.CDLL:
         "plife_batch_destroy": [vp],
         "plife_batch_add_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32, vp, vp, vp,
                                   ctypes.POINTER(i32)],
+        "plife_batch_add_generated_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32,
+                                            ctypes.c_uint64, ctypes.POINTER(i32)],
         "plife_batch_remove_world": [vp, i32],
         "plife_batch_set_world_settings": [vp, i32, ctypes.POINTER(PlifeSettings), i32, vp],
         "plife_batch_update": [vp, i32],

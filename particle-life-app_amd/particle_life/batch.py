@@ -62,6 +62,20 @@ class Batch:
         self._n[wid.value] = n
         return wid.value
 
+    def add_generated(self, *, n: int, seed: int, matrix, rmax, friction, dt, beta=0.3, force=1.0,
+                      wrap=True, dim=2) -> int:
+        """Adds a world whose particles are generated on the device from `seed`
+        (the state ``worlds.uniform_world(seed, n, ...)`` builds on the host)."""
+        m = np.ascontiguousarray(matrix, dtype=np.float64)
+        s = _settings(rmax=rmax, friction=friction, dt=dt, beta=beta, force=force, wrap=wrap, dim=dim)
+        wid = ctypes.c_int32()
+        check(self._lib.plife_batch_add_generated_world(self._h, ctypes.byref(s), int(m.shape[0]),
+                                                        m.ctypes.data, int(n),
+                                                        ctypes.c_uint64(int(seed) & (2**64 - 1)),
+                                                        ctypes.byref(wid)))
+        self._n[wid.value] = int(n)
+        return wid.value
+
     def add_world(self, w) -> int:
         """Adds a ``worlds.World``."""
         return self.add(matrix=w.matrix, pos=w.pos, vel=w.vel, types=w.types, rmax=w.rmax,

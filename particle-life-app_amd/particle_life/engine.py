@@ -62,6 +62,13 @@ class Engine:
                                             types.ctypes.data))
         self.n = n
 
+    def generate(self, n: int, seed: int) -> None:
+        """Replaces the particles with n generated on the device from `seed`
+        (uniform positions and types, v = 0): the same state as
+        ``worlds.uniform_world(seed, n, ...)`` builds on the host."""
+        check(self._lib.plife_generate_particles(self._h, int(n), ctypes.c_uint64(int(seed) & (2**64 - 1))))
+        self.n = int(n)
+
     def load_world(self, w) -> None:
         self.configure_world(w)
         self.set_state(w.pos, w.vel, w.types)
