@@ -243,6 +243,19 @@ __device__ void home_global(const ForceArgs &a, const WorldDev &w, const double 
 // when the neighbourhood is denser than the staging buffer.  Workgroups with
 // more parts or staged cells than the cell table holds take the global path.
 constexpr int kMaxParts = 4;
+constexpr int kDealKeys = 5;     // 4 position groups + the inactive threads of a short block
+
+// Position group of a home inside its cell: y quartile (2D) or y/z halves (3D).
+template <int DIM>
+__device__ __forceinline__ int deal_key(uint32_t y, uint32_t z, int hrow, int nx, double cs)
+{
+    const float inv = (float)(kInvFix / cs);
+    const int cy = hrow % nx;
+    const float fy = (float)y * inv - (float)cy;           // ~[0, 1) within the cell
+    if constexpr (DIM == 2) return min(3, max(0, (int)(fy * 4.0f)));
+    const float fz = (float)z * inv - (float)(hrow / nx);
+    return min(1, max(0, (int)(fz * 2.0f))) * 2 + min(1, max(0, (int)(fy * 2.0f)));
+}
 
 struct ForceLds {
     size_t m, sxy, lists, sz, sid, off, gs, st, total;
@@ -279,6 +292,8 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     __shared__ int s_part[kMaxParts][3];      // row, first column, last column
     __shared__ uint32_t s_cnt[NW + 1];
     __shared__ unsigned long long s_pairs[NW];
+    __shared__ uint32_t s_qc[NW][kDealKeys];
+    __shared__ uint32_t s_deal[BS];
 
     const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells);
     double *M = reinterpret_cast<double *>(smem + L.m);
@@ -298,26 +313,23 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const uint32_t cb = w.cell_base;
     const int g0 = (int)bw.y;
     const int nact = min(BS, w.off + w.n - g0);
-    const bool active = tid < nact;
-    const int g = g0 + (active ? tid : nact - 1);
 
     if constexpr (MLDS)
         for (int t = tid; t < w.ntypes * w.ntypes; t += BS) M[t] = a.matrix[w.moff + t];
 
-    // this thread's home: grid row (2D: cy; 3D: cy + cz*nx) and column
-    const uint32_t hkey = a.skeys[g] - cb;
-    const int hrow = (int)(hkey / (uint32_t)nx);
-    const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
-    const uint32_t ho = a.perm[g];
-    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
-    const int htype = a.type[ho];
-    const int mrow = htype * w.ntypes;
+    // Thread tid first stands for home g0 + tid (sorted order), to find the
+    // parts: grid rows (2D: cy; 3D: cy + cz*nx) and columns of the homes.
+    const bool active0 = tid < nact;
+    const int gs0 = g0 + (active0 ? tid : nact - 1);
+    const uint32_t hkey0 = a.skeys[gs0] - cb;
+    const int hrow0 = (int)(hkey0 / (uint32_t)nx);
+    const int hcx0 = (int)(hkey0 - (uint32_t)hrow0 * (uint32_t)nx);
 
     // parts = distinct rows among the homes (rows are non-decreasing with tid)
-    const int prow = tid > 0 ? (int)((a.skeys[g - 1] - cb) / (uint32_t)nx) : -1;
-    const bool pstart = active && (tid == 0 || prow != hrow);
-    const int nrow_next = (tid + 1 < nact) ? (int)((a.skeys[g + 1] - cb) / (uint32_t)nx) : -1;
-    const bool pend = active && (tid + 1 == nact || nrow_next != hrow);
+    const int prow = tid > 0 ? (int)((a.skeys[gs0 - 1] - cb) / (uint32_t)nx) : -1;
+    const bool pstart = active0 && (tid == 0 || prow != hrow0);
+    const int nrow_next = (tid + 1 < nact) ? (int)((a.skeys[gs0 + 1] - cb) / (uint32_t)nx) : -1;
+    const bool pend = active0 && (tid + 1 == nact || nrow_next != hrow0);
     const uint64_t wstarts = __ballot(pstart);
     if (lane == 0) s_cnt[wv] = (uint32_t)__popcll(wstarts);
     __syncthreads();
@@ -328,13 +340,50 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         nparts += s_cnt[w];
     }
     const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    const int mypart = (int)(before + __popcll(wstarts & le)) - 1;
-    if (pstart && mypart < kMaxParts) {
-        s_part[mypart][0] = hrow;
-        s_part[mypart][1] = hcx;
+    const int mypart0 = (int)(before + __popcll(wstarts & le)) - 1;
+    if (pstart && mypart0 < kMaxParts) {
+        s_part[mypart0][0] = hrow0;
+        s_part[mypart0][1] = hcx0;
     }
-    if (pend && mypart < kMaxParts) s_part[mypart][2] = hcx;
+    if (pend && mypart0 < kMaxParts) s_part[mypart0][2] = hcx0;
+
+    // Deal the homes to threads grouped by where they sit in their cell (y
+    // quartile in 2D, y/z halves in 3D; stable within a group).  A home's
+    // accepted candidates bunch in the stencil rows nearest to it, so lanes
+    // of one wave then fill their candidate lists at similar rates and list
+    // flushes keep more lanes busy.  Each home's own work is unchanged.
+    const int dkey = active0 ? deal_key<DIM>(a.sy[a.perm[gs0]], DIM == 3 ? a.sz[a.perm[gs0]] : 0u,
+                                             hrow0, nx, w.cs)
+                             : kDealKeys - 1;
+    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t myrank = 0;
+#pragma unroll
+    for (int q = 0; q < kDealKeys; ++q) {
+        const uint64_t b = __ballot(dkey == q);
+        if (lane == 0) s_qc[wv][q] = (uint32_t)__popcll(b);
+        if (dkey == q) myrank = (uint32_t)__popcll(b & lt);
+    }
     __syncthreads();
+    for (int q = 0; q < kDealKeys; ++q)
+#pragma unroll
+        for (int v = 0; v < NW; ++v)
+            myrank += (q < dkey || (q == dkey && v < wv)) ? s_qc[v][q] : 0u;
+    s_deal[myrank] = (uint32_t)tid | ((uint32_t)max(0, min(mypart0, 255)) << 16);
+    __syncthreads();
+
+    // this thread's home after the deal
+    const uint32_t dealt = s_deal[tid];
+    const int th = (int)(dealt & 0xFFFFu);
+    const int mypart = (int)(dealt >> 16);
+    const bool active = th < nact;
+    const int g = g0 + (active ? th : nact - 1);
+    const uint32_t hkey = a.skeys[g] - cb;
+    const int hrow = (int)(hkey / (uint32_t)nx);
+    const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
+    const uint32_t ho = a.perm[g];
+    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
+    const int htype = a.type[ho];
+    const int mrow = htype * w.ntypes;
 
     int p_row[kMaxParts], p_ca[kMaxParts], p_w[kMaxParts], p_base[kMaxParts + 1];
     p_base[0] = 0;
