@@ -684,7 +684,8 @@ int record(plife_physics *h, PhaseEvents *ev, int k)
 // occupancy (sum over cells of count^2, divided by N: the mean size of the
 // cell a particle sits in) measured by an earlier step:
 //   * uniform worlds: 256 homes per workgroup and a staging buffer that holds
-//     the whole neighbourhood (one window), list capacity rising with density;
+//     the whole neighbourhood (one window); 24-entry candidate lists where
+//     cells are dense enough for the LDS to limit occupancy, 32 where sparse;
 //   * clustered worlds (occupancy far above the mean N / cells): 128 homes per
 //     workgroup and a small staging buffer, so that many workgroups share a CU
 //     while the dense neighbourhoods stream through it in windows.
@@ -701,7 +702,7 @@ ForcePlan plan_force(const plife_physics *h)
     ForcePlan p;
     p.bs = t.block_homes ? t.block_homes : (clustered ? 128 : 256);
     p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))
-                            : (clustered || occ >= 12.0 ? 32 : 16);
+                            : (clustered || occ < 12.0 ? 32 : 24);
     p.lcap = std::min(p.lcap, 16384 / p.bs);             // lists stay within 32 KB of LDS
     // a workgroup of bs homes spans ~bs/avg + 1 cells of a row; it stages NR
     // rows of (span + 2) cells.  Clustered worlds keep a short cell table: the
