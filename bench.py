@@ -47,6 +47,7 @@ def parse():
                         "c5: a Poisson stream of jobs served by continuous batching")
     p.add_argument("--jobs", type=int, default=200, help="c5: jobs in the trace")
     p.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
+    p.add_argument("--quantum", type=int, default=32, help="c5: most steps per batch update")
     p.add_argument("--worlds", type=int, default=512, help="c4: number of worlds")
     p.add_argument("--no-plain", action="store_true",
                    help="c4/c5: skip the comparison with the plain (one world per handle) path")
@@ -198,7 +199,7 @@ def bench_serve(args) -> int:
     jobs = worlds.config5_jobs(args.jobs, rate=args.rate)
     for j in jobs:
         j["world"] = worlds.job_world(j)          # clients bring their initial state
-    served = serve(jobs)
+    served = serve(jobs, quantum=args.quantum)
     s = served.summary()
     out = {
         "metric": "served job latency p99 (continuous batching)",
@@ -213,6 +214,8 @@ def bench_serve(args) -> int:
         "job_steps_per_s": s["job_steps_per_s"],
         "particle_steps_per_s": s["particle_steps_per_s"],
         "updates": s["updates"],
+        "host_s": s["host_s"],
+        "quantum": args.quantum,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

@@ -209,6 +209,7 @@ struct plife_physics {
     // HIP graph of two consecutive steps (buffer roles repeat every 2 steps)
     hipGraphExec_t gexec = nullptr;
     std::vector<unsigned char> gkey;
+    std::vector<unsigned char> seen_key;   // configuration of the previous update
 
     // launch tuning (0 = automatic), the current plan, what the last launch used
     plife_tuning tune{};
@@ -939,12 +940,24 @@ void count_steps(plife_physics *h, int k)
     for (WorldHost &w : h->worlds) w.steps += k;
 }
 
-// nsteps steps: pairs of steps replay a captured graph, the rest launch directly.
+// nsteps steps: pairs of steps replay a captured graph, the rest launch
+// directly.  A graph is captured once the same configuration comes back on a
+// second update (or when one update runs many steps); a configuration that
+// changes every update (worlds joining and leaving) launches directly, as a
+// capture would not be replayed enough to pay for itself.
 int run_steps(plife_physics *h, int nsteps, bool honour_stop)
 {
     int k = 0, rc;
     if ((rc = prepare_step(h))) return rc;
-    if (nsteps >= 2 && !h->tune.direct_launch) {
+    bool use_graph = nsteps >= 2 && !h->tune.direct_launch;
+    if (use_graph) {
+        std::vector<unsigned char> key = graph_key(h);
+        if (!h->gexec || key != h->gkey) {
+            if (key != h->seen_key && nsteps < 64) use_graph = false;
+            h->seen_key = key;
+        }
+    }
+    if (use_graph) {
         std::vector<unsigned char> key = graph_key(h);
         if (!h->gexec || key != h->gkey) {
             if ((rc = capture_two_steps(h))) return rc;
@@ -971,9 +984,11 @@ int time_steps(plife_physics *h, int nsteps, double *ms)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    // build (or reuse) the step graph outside the timed region
+    // build (or reuse) the step graph outside the timed region: the first
+    // update of a new configuration launches directly, the second captures
     if ((rc = prepare_step(h))) return rc;
-    if (nsteps >= 2 && !h->tune.direct_launch && (!h->gexec || graph_key(h) != h->gkey))
+    for (int i = 0; i < 2 && nsteps >= 2 && !h->tune.direct_launch &&
+                    (!h->gexec || graph_key(h) != h->gkey); ++i)
         if ((rc = run_steps(h, 2, false))) return rc;
     HIP_TRY(hipEventRecord(e0, h->stream));
     rc = run_steps(h, nsteps, false);

@@ -66,6 +66,8 @@ class ServeStats:
     updates: int = 0
     batch_steps: int = 0
     peak_bytes: int = 0
+    host_s: dict = field(default_factory=lambda: {"admit": 0.0, "enqueue": 0.0, "wait": 0.0,
+                                                   "retire": 0.0, "idle": 0.0})
 
     def latencies(self) -> np.ndarray:
         return np.array([r.latency for r in self.results])
@@ -85,6 +87,7 @@ class ServeStats:
             "particle_steps_per_s": psteps / self.wall_s if self.wall_s else 0.0,
             "updates": self.updates,
             "peak_bytes": self.peak_bytes,
+            "host_s": dict(self.host_s),
         }
 
 
@@ -108,6 +111,7 @@ def serve(jobs: list[dict], device: int = 0, budget_bytes: float = 64e9, quantum
             while pending and pending[0]["arrival"] <= now:
                 queue.append(pending.popleft())
             # admission in arrival order while the footprint fits
+            ta = clock()
             while queue:
                 j = queue[0]
                 w = j["world"]
@@ -124,9 +128,12 @@ def serve(jobs: list[dict], device: int = 0, budget_bytes: float = 64e9, quantum
                 active[w.dim].append(_Active(res, wid, j["steps"], need))
                 used += need
                 st.peak_bytes = max(st.peak_bytes, used)
+            tb = clock()
+            st.host_s["admit"] += tb - ta
             if not (active[2] or active[3]):
                 if pending:                    # idle until the next arrival
                     time.sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
+                st.host_s["idle"] += clock() - tb
                 continue
             stepped = []
             for dim, act in active.items():
@@ -137,8 +144,13 @@ def serve(jobs: list[dict], device: int = 0, budget_bytes: float = 64e9, quantum
                 st.batch_steps += k
                 stepped.append((dim, k))
             st.updates += 1
-            for dim, k in stepped:
+            tc = clock()
+            st.host_s["enqueue"] += tc - tb
+            for dim, _ in stepped:
                 batches[dim].sync()
+            td = clock()
+            st.host_s["wait"] += td - tc
+            for dim, k in stepped:
                 keep = []
                 for a in active[dim]:
                     a.left -= k
@@ -152,6 +164,7 @@ def serve(jobs: list[dict], device: int = 0, budget_bytes: float = 64e9, quantum
                     used -= a.bytes
                     st.results.append(a.res)
                 active[dim] = keep
+            st.host_s["retire"] += clock() - td
     finally:
         for b in batches.values():
             b.close()
