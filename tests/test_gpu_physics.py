@@ -1,0 +1,91 @@
+"""The drop-in mirror of the reference's Physics API on the device: update()
+against the oracle, the reference's setters (setParticleCount, setMatrixSize,
+ensureTypes, setPositions), snapshots and save-state round trips."""
+from __future__ import annotations
+
+import io
+
+import numpy as np
+import pytest
+
+import oracle
+from parity_util import VEL_ABS, VEL_REL
+
+pytestmark = pytest.mark.gpu
+
+
+def _physics(n=5000, seed=5):
+    from particle_life import (DefaultMatrixGenerator, DefaultPositionSetter, DefaultTypeSetter,
+                               GpuPhysics, ParticleLifeAccelerator)
+
+    rng = np.random.default_rng(seed)
+    return GpuPhysics(ParticleLifeAccelerator(0.3), DefaultPositionSetter(rng),
+                      DefaultMatrixGenerator(rng), DefaultTypeSetter(rng), particleCount=n)
+
+
+def test_update_matches_oracle():
+    print("\n[physics] case=update_vs_oracle", flush=True)
+    ph = _physics()
+    ph.settings.dt = 0.02                      # the app's per-frame dt (app/Main.java:292-295)
+    ph.update(0)                               # upload: the device rounds positions to 2^-32
+    p0 = ph.particles
+    s = ph.settings
+    S = oracle.Settings(matrix=s.matrix.to_array(), rmax=s.rmax, friction=s.friction, dt=s.dt,
+                        beta=ph.accelerator.beta, force=s.force, wrap=s.wrap, dim=2)
+    ref = oracle.step(S, p0.position, p0.velocity, p0.type, nthreads=8)
+    for _ in range(1):
+        ph.update()
+    p1 = ph.particles
+    np.testing.assert_array_equal(p1.ids, ref["ids"], err_msg="particle order")
+    np.testing.assert_array_equal(p1.type, ref["types"])
+    dv = np.abs(p1.velocity - ref["vel"])
+    assert not (dv > VEL_ABS + VEL_REL * np.abs(ref["vel"])).any()
+    d = np.abs(p1.position - ref["pos"])
+    d = np.minimum(d, 1.0 - d)
+    assert d.max() < 2.0 ** -32 + s.dt * (VEL_ABS + VEL_REL * np.abs(ref["vel"]).max())
+    ph.kill()
+
+
+def test_reference_setters_and_snapshot():
+    print("\n[physics] case=setters", flush=True)
+    ph = _physics(3000)
+    ph.update(2)
+    ph.setParticleCount(4500)
+    assert len(ph.particles) == 4500
+    ph.update(2)
+    ph.setMatrixSize(4)                        # shrinking runs ensureTypes
+    assert ph.particles.type.max() < 4
+    ph.update(2)
+    ph.setParticleCount(1000)
+    ph.setPositions()
+    assert (ph.particles.velocity == 0).all()
+    ph.update(3)
+    snap = ph.snapshot()
+    assert snap.positions.shape == (3000,) and snap.particleCount == 1000
+    assert snap.typeCount.sum() == 1000 and len(snap.typeCount) == 4
+    ph.kill()
+
+
+def test_save_load_round_trip_steps_identically():
+    from particle_life import saves
+
+    print("\n[physics] case=save_load", flush=True)
+    a = _physics(4000, seed=9)
+    a.settings.dt = 0.02
+    a.update(3)
+    buf = io.BytesIO()
+    saves.save_physics(a, buf)
+    buf.seek(0)
+    b = _physics(10, seed=10)
+    b.settings.dt = 0.02
+    saves.load_physics(b, buf)
+    np.testing.assert_array_equal(b.particles.position, a.particles.position)
+    np.testing.assert_array_equal(b.particles.velocity, a.particles.velocity)
+    assert b.settings.matrix == a.settings.matrix and b.settings.rmax == a.settings.rmax
+    a.update(2)
+    b.update(2)
+    pa, pb = a.particles, b.particles
+    np.testing.assert_array_equal(pa.position, pb.position)
+    np.testing.assert_array_equal(pa.velocity, pb.velocity)
+    a.kill()
+    b.kill()
