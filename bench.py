@@ -273,6 +273,20 @@ def main() -> int:
 
     # per-phase timing (events around each phase) on further steps
     prof = eng.profile(args.profile_steps)
+
+    # the app's path: one update and a full state download (PhysicsSnapshot)
+    # per frame, host-visible wall time
+    frames = 5
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        eng.step(1)
+        eng.state()
+    frame_ms = (time.perf_counter() - t0) * 1e3 / frames
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        eng.state()
+    get_state_ms = (time.perf_counter() - t0) * 1e3 / frames
     pairs = prof["pairs"]
     steps_per_s = args.steps / (ms / 1e3)
     value = pairs_last * steps_per_s
@@ -317,6 +331,9 @@ def main() -> int:
             "n": w.n, "dim": w.dim, "ntypes": w.ntypes, "rmax_unit_box": w.rmax, "dt": w.dt,
         },
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
+        "app_path": {"ms_per_frame": frame_ms, "get_state_ms": get_state_ms,
+                     "how": "update(1) + plife_get_particles of positions, velocities, types and "
+                            "ids as double[3N]/int[N] per frame, wall clock"},
         "roofline": {
             "kernel": f"k_force<{w.dim}, wrap, no-debug>",
             "bound": "valu",

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -190,6 +191,8 @@ struct plife_physics {
     unsigned long long *d_pairs = nullptr;
     size_t pairs_cap = 0;
     uint64_t *stat_host = nullptr;     // pinned copy of d_pairs[1], written every step
+    unsigned char *pin = nullptr;      // pinned staging of state downloads
+    size_t pin_cap = 0;
     double occupancy = 0.0;            // particle-weighted cell occupancy seen by the planner
     bool clustered = false;            // the planner's regime (with hysteresis)
 
@@ -1168,39 +1171,80 @@ int generate_particles(plife_physics *h, int off, int n, uint64_t seed, int ntyp
     return PLIFE_OK;
 }
 
-// Downloads slots [off, off + n) of the current buffers.
+// Runs f(lo, hi) over [0, n) on up to 8 host threads (one for small n).
+template <typename F>
+void parallel_chunks(int n, F f)
+{
+    const int nt = n >= (1 << 16) ? (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    if (nt <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    const int chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) ts.emplace_back(f, lo, hi);
+    }
+    for (auto &t : ts) t.join();
+}
+
+// Downloads slots [off, off + n) of the current buffers: one DMA per array
+// into pinned staging, then the conversion to the caller's FP64 triples on
+// host threads.
 int download_particles(plife_physics *h, int off, int n, double *pos, double *vel, int32_t *types,
                        int32_t *ids, bool d3)
 {
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (n == 0) return PLIFE_OK;
+    const size_t b = (size_t)n * 4;
+    const size_t need = 8 * b;
+    if (need > h->pin_cap) {
+        if (h->pin) (void)hipHostFree(h->pin);
+        h->pin = nullptr;
+        h->pin_cap = 0;
+        const hipError_t e = hipHostMalloc((void **)&h->pin, need, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            h->pin = nullptr;
+            return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", need, hipGetErrorString(e));
+        }
+        h->pin_cap = need;
+    }
     const PosBuf P = h->P(h->pc);
     const AuxBuf A = h->A(h->ac);
-    const size_t b = (size_t)n * 4;
+    uint32_t *sx = (uint32_t *)h->pin, *sy = sx + n, *sz = sy + n;
+    float *svx = (float *)(sz + n), *svy = svx + n, *svz = svy + n;
+    int32_t *st = (int32_t *)(svz + n), *sid = st + n;
+    const hipMemcpyKind k = hipMemcpyDeviceToHost;
     if (pos) {
-        std::vector<uint32_t> x(n), y(n), z(n);
-        HIP_TRY(hipMemcpy(x.data(), P.x + off, b, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(y.data(), P.y + off, b, hipMemcpyDeviceToHost));
-        if (d3) HIP_TRY(hipMemcpy(z.data(), P.z + off, b, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) {
-            pos[3 * i] = (double)x[i] * kInvFix;
-            pos[3 * i + 1] = (double)y[i] * kInvFix;
-            pos[3 * i + 2] = d3 ? (double)z[i] * kInvFix : 0.0;
-        }
+        HIP_TRY(hipMemcpyAsync(sx, P.x + off, b, k, h->stream));
+        HIP_TRY(hipMemcpyAsync(sy, P.y + off, b, k, h->stream));
+        if (d3) HIP_TRY(hipMemcpyAsync(sz, P.z + off, b, k, h->stream));
     }
     if (vel) {
-        std::vector<float> x(n), y(n), z(n);
-        HIP_TRY(hipMemcpy(x.data(), A.vx + off, b, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(y.data(), A.vy + off, b, hipMemcpyDeviceToHost));
-        if (d3) HIP_TRY(hipMemcpy(z.data(), A.vz + off, b, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) {
-            vel[3 * i] = x[i];
-            vel[3 * i + 1] = y[i];
-            vel[3 * i + 2] = d3 ? (double)z[i] : 0.0;
-        }
+        HIP_TRY(hipMemcpyAsync(svx, A.vx + off, b, k, h->stream));
+        HIP_TRY(hipMemcpyAsync(svy, A.vy + off, b, k, h->stream));
+        if (d3) HIP_TRY(hipMemcpyAsync(svz, A.vz + off, b, k, h->stream));
     }
-    if (types) HIP_TRY(hipMemcpy(types, A.type + off, b, hipMemcpyDeviceToHost));
-    if (ids) HIP_TRY(hipMemcpy(ids, A.id + off, b, hipMemcpyDeviceToHost));
+    if (types) HIP_TRY(hipMemcpyAsync(st, A.type + off, b, k, h->stream));
+    if (ids) HIP_TRY(hipMemcpyAsync(sid, A.id + off, b, k, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    parallel_chunks(n, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            if (pos) {
+                pos[3 * i] = (double)sx[i] * kInvFix;
+                pos[3 * i + 1] = (double)sy[i] * kInvFix;
+                pos[3 * i + 2] = d3 ? (double)sz[i] * kInvFix : 0.0;
+            }
+            if (vel) {
+                vel[3 * i] = svx[i];
+                vel[3 * i + 1] = svy[i];
+                vel[3 * i + 2] = d3 ? (double)svz[i] : 0.0;
+            }
+        }
+        if (types) std::memcpy(types + lo, st + lo, (size_t)(hi - lo) * 4);
+        if (ids) std::memcpy(ids + lo, sid + lo, (size_t)(hi - lo) * 4);
+    });
     return PLIFE_OK;
 }
 
@@ -1241,6 +1285,7 @@ void destroy_handle(plife_physics *h)
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
     if (h->stat_host) (void)hipHostFree(h->stat_host);
+    if (h->pin) (void)hipHostFree(h->pin);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 

@@ -109,10 +109,10 @@ class Batch:
     # state
     def state(self, wid: int) -> dict:
         n = self._n.get(int(wid), 0)          # an unknown id is reported by the library
-        pos = np.zeros((n, 3))
-        vel = np.zeros((n, 3))
-        typ = np.zeros(n, np.int32)
-        ids = np.zeros(n, np.int32)
+        pos = np.empty((n, 3))
+        vel = np.empty((n, 3))
+        typ = np.empty(n, np.int32)
+        ids = np.empty(n, np.int32)
         check(self._lib.plife_batch_get_world(self._h, int(wid), n, pos.ctypes.data, vel.ctypes.data,
                                               typ.ctypes.data, ids.ctypes.data))
         return {"pos": pos, "vel": vel, "types": typ, "ids": ids}

@@ -75,10 +75,10 @@ class Engine:
 
     def state(self) -> dict:
         n = self.n
-        pos = np.zeros((n, 3))
-        vel = np.zeros((n, 3))
-        typ = np.zeros(n, np.int32)
-        ids = np.zeros(n, np.int32)
+        pos = np.empty((n, 3))
+        vel = np.empty((n, 3))
+        typ = np.empty(n, np.int32)
+        ids = np.empty(n, np.int32)
         check(self._lib.plife_get_particles(self._h, n, pos.ctypes.data, vel.ctypes.data,
                                             typ.ctypes.data, ids.ctypes.data))
         return {"pos": pos, "vel": vel, "types": typ, "ids": ids}
