@@ -23,6 +23,8 @@
 // below 2^-21 relative), so neighbour sets are exactly the reference's.
 #pragma once
 
+#include <type_traits>
+
 #include "plife_kernels.hpp"
 
 namespace plife {
@@ -135,6 +137,40 @@ __device__ __forceinline__ bool screen(uint32_t hx, uint32_t hy, uint32_t hz, ui
         if constexpr (DIM == 3) pass &= ((dzi >= 0) == (qz >= hz));
     }
     return pass;
+}
+
+// Periodic distance in cells between columns (rows, layers) a and b of n.
+__device__ __forceinline__ int periodic_cells(int a, int b, int n)
+{
+    int d = a - b;
+    d = d < 0 ? -d : d;
+    d %= n;
+    return min(d, n - d);
+}
+
+// 16-bit relative coordinate of fixed-point p against the origin o (floor of
+// the periodic int32 difference over 2^sh), as the low 16 bits of a u32.
+__device__ __forceinline__ uint32_t rel16(uint32_t p, uint32_t o, int sh)
+{
+    return (uint32_t)((int)(p - o) >> sh) & 0xFFFFu;
+}
+
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+// Phase A screen on packed 16-bit relative coordinates (x | y << 16, z apart):
+// packed subtract and a saturating dot product against the quantised bound.
+// Each coordinate is at most one unit off its exact quotient, so the bound
+// (Rq + 2)^2 keeps every pair with |D| <= rmax (1 + 2^-20).
+template <int DIM>
+__device__ __forceinline__ bool screen16(uint32_t q, uint32_t h, uint32_t qz, uint32_t hz, int bq)
+{
+    const short2v d = __builtin_bit_cast(short2v, q) - __builtin_bit_cast(short2v, h);
+    int zz = 0;
+    if constexpr (DIM == 3) {
+        const int dz = (int)(short)(uint16_t)(qz - hz);
+        zz = dz * dz;
+    }
+    return __builtin_amdgcn_sdot2(d, d, zz, true) <= bq;
 }
 
 __device__ __forceinline__ int wrap_cell(int c, int n)
@@ -258,7 +294,7 @@ __device__ __forceinline__ int deal_key(uint32_t y, uint32_t z, int hrow, int nx
 }
 
 struct ForceLds {
-    size_t m, sxy, lists, sz, sid, off, gs, st, total;
+    size_t m, sxy, sq, lists, sz, sqz, sid, off, gs, st, total;
 };
 
 __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int ntypes, bool m_in_lds,
@@ -268,9 +304,12 @@ __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int nt
     size_t o = 0;
     L.m = o;     o += m_in_lds ? (size_t)ntypes * ntypes * 8 : 0;
     L.sxy = o;   o += (size_t)cap * 8;
+    L.sq = o;    o += dim == 3 ? (size_t)cap * 4 : 0;     // 16-bit screen (3D only)
     L.lists = o; o += (size_t)lcap * bs * 2;
     o = (o + 3) & ~(size_t)3;
     L.sz = o;    o += dim == 3 ? (size_t)cap * 4 : 0;
+    L.sqz = o;   o += dim == 3 ? (size_t)cap * 2 : 0;
+    o = (o + 3) & ~(size_t)3;
     L.sid = o;   o += debug ? (size_t)cap * 4 : 0;
     L.off = o;   o += (size_t)(max_cells + 1) * 4;
     L.gs = o;    o += (size_t)max_cells * 4;
@@ -298,6 +337,8 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells);
     double *M = reinterpret_cast<double *>(smem + L.m);
     uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
+    uint32_t *sq = reinterpret_cast<uint32_t *>(smem + L.sq);
+    uint16_t *sqz = reinterpret_cast<uint16_t *>(smem + L.sqz);
     uint16_t *lists = reinterpret_cast<uint16_t *>(smem + L.lists);
     uint32_t *szz = reinterpret_cast<uint32_t *>(smem + L.sz);
     uint32_t *sid = reinterpret_cast<uint32_t *>(smem + L.sid);
@@ -398,6 +439,42 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const int NC = p_base[kMaxParts];
     bool use_lds = !w.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
 
+    // 3D phase A screens on 16-bit positions relative to the workgroup's first
+    // home: rel = (int32)(p - origin) >> sh, with sh chosen so that every
+    // staged particle and home lies within 2^14 units of the origin (periodic
+    // distance, in cells, bounded from the parts' staged ranges).  Relative
+    // differences equal the minimum-image differences only while everything
+    // lies within a quarter box of the origin (two such points are less than
+    // half a box apart); wider workgroups (coarse grids, sparse parts) screen
+    // on the exact 32-bit differences instead.
+    const uint32_t go = a.perm[g0];
+    const uint32_t ox = a.sx[go], oy = a.sy[go], oz = DIM == 3 ? a.sz[go] : 0u;
+    int sh = 0, bq = 0;
+    bool narrow = false;
+    {
+        const uint32_t okey = a.skeys[g0] - cb;
+        const int orow = (int)(okey / (uint32_t)nx);
+        const int ocx = (int)(okey - (uint32_t)orow * (uint32_t)nx);
+        const int ocy = orow % nx, ocz = DIM == 3 ? orow / nx : 0;
+        int ext = 0;
+#pragma unroll
+        for (int p = 0; p < kMaxParts; ++p) {
+            if (p >= (int)nparts) break;
+            const int cy = p_row[p] % nx, cz = DIM == 3 ? p_row[p] / nx : 0;
+            ext = max(ext, periodic_cells(p_ca[p] - 1, ocx, nx) + p_w[p]);
+            ext = max(ext, periodic_cells(cy, ocy, nx) + 2);
+            if (DIM == 3) ext = max(ext, periodic_cells(cz, ocz, nx) + 2);
+        }
+        // + 3 cells: positions inside the particle's and the origin's cells
+        // (the last cell of a row can be up to twice as wide)
+        const double span_units = (double)(ext + 3) * w.cs * kFix;
+        narrow = span_units < 0.25 * kFix;
+        while (sh < 31 && span_units >= 16384.0 * (double)(1u << sh)) ++sh;
+        // |D|^2 <= rmax^2 implies sum (D_i / 2^sh)^2 in quantised units <= (Rq + 2)^2
+        const double rq = w.cs * kFix * (1.0 + 0x1p-20) / (double)(1u << sh) + 2.0;
+        bq = (int)fmin(2147483647.0, floor(rq * rq) + 1.0);
+    }
+
     // staged cells: start and count, then an exclusive prefix in LDS (carry ends
     // as the staged particle count, the same in every thread)
     uint32_t carry = 0;
@@ -457,6 +534,8 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         const bool q2y = (cy == nx - 1) && stencil_origin(hy, w.cs) == nx;
         const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, w.cs) == nx;
         const int j_lo = jh - 1 + (q2x ? 1 : 0);
+        const uint32_t hq = rel16(hx, ox, sh) | (rel16(hy, oy, sh) << 16);
+        const uint32_t hqz = DIM == 3 ? rel16(hz, oz, sh) : 0u;
 
         // The staged neighbourhood is S particles; it is brought into LDS in
         // windows of at most cap (one window for ordinary densities, several
@@ -473,8 +552,14 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     else hi = mid;
                 }
                 const uint32_t q = a.perm[gs[lo] + ((uint32_t)sg - off[lo])];
-                sxy[s] = make_uint2(a.sx[q], a.sy[q]);
-                if constexpr (DIM == 3) szz[s] = a.sz[q];
+                const uint32_t qx = a.sx[q], qy = a.sy[q];
+                sxy[s] = make_uint2(qx, qy);
+                if constexpr (DIM == 3) {
+                    const uint32_t qz = a.sz[q];
+                    szz[s] = qz;
+                    sq[s] = rel16(qx, ox, sh) | (rel16(qy, oy, sh) << 16);
+                    sqz[s] = (uint16_t)rel16(qz, oz, sh);
+                }
                 if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
                 st[s] = (uint8_t)a.type[q];
             }
@@ -487,37 +572,62 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     const uint2 q = sxy[k];
                     const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
                     const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
+                    if constexpr (!WRAP) {
+                        // plain difference of the clamped world: |D| >= 1/2 unless the
+                        // int32 difference has the sign of the unsigned order
+                        bool ok = ((dxi >= 0) == (q.x >= hx)) & ((dyi >= 0) == (q.y >= hy));
+                        if (DIM == 3) ok &= ((dzi >= 0) == (szz[k] >= hz));
+                        if (!ok) continue;
+                    }
                     const int idx = mrow + (int)st[k];
                     const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
                     pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
                 }
                 cnt = 0;
             };
+            // the row scan, instantiated for the 16-bit and the 32-bit screen
+            auto scan_rows = [&](auto narrow_tag) {
+                constexpr bool NARROW = decltype(narrow_tag)::value;
 #pragma unroll 1
-            for (int r = 0; r < NR; ++r) {
-                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-                const bool skip = !active || (q2y && oy == -1) || (q2z && oz == -1);
-                const int rb = pb + r * pw;
-                const int klo = max((int)off[rb + j_lo], w0) - w0;
-                const int khi = min((int)off[rb + jh + 2], w0 + wn) - w0;
-                const int len = skip ? 0 : max(0, khi - klo);
-                const int K = wave_max(len);
-                for (int i = 0; i < K; i += U) {
-                    if (__any(cnt > a.lcap - U)) flush();
+                for (int r = 0; r < NR; ++r) {
+                    const int oyr = (r % 3) - 1, ozr = DIM == 3 ? (r / 3) - 1 : 0;
+                    const bool skip = !active || (q2y && oyr == -1) || (q2z && ozr == -1);
+                    const int rb = pb + r * pw;
+                    const int klo = max((int)off[rb + j_lo], w0) - w0;
+                    const int khi = min((int)off[rb + jh + 2], w0 + wn) - w0;
+                    const int len = skip ? 0 : max(0, khi - klo);
+                    const int K = wave_max(len);
+                    for (int i = 0; i < K; i += U) {
+                        if (__any(cnt > a.lcap - U)) flush();
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int ii = i + u;
-                        if (ii < len) {
-                            const int k = klo + ii;
-                            const uint2 q = sxy[k];
-                            const uint32_t qz = DIM == 3 ? szz[k] : 0u;
-                            if (screen<DIM, WRAP>(hx, hy, hz, q.x, q.y, qz, w.r2hi)) {
-                                lists[cnt * BS + tid] = (uint16_t)k;
-                                ++cnt;
+                        for (int u = 0; u < U; ++u) {
+                            const int ii = i + u;
+                            if (ii < len) {
+                                const int k = klo + ii;
+                                bool pass;
+                                if constexpr (NARROW) {
+                                    pass = screen16<DIM>(sq[k], hq, DIM == 3 ? sqz[k] : 0u, hqz, bq);
+                                } else {
+                                    const uint2 q = sxy[k];
+                                    pass = screen<DIM, WRAP>(hx, hy, hz, q.x, q.y,
+                                                             DIM == 3 ? szz[k] : 0u, w.r2hi);
+                                }
+                                if (pass) {
+                                    lists[cnt * BS + tid] = (uint16_t)k;
+                                    ++cnt;
+                                }
                             }
                         }
                     }
                 }
+            };
+            // 2D screens on the 32-bit differences: its FP32 screen is short and
+            // the 16-bit copy would cost LDS occupancy; 3D gains 2/3 of its screen
+            if constexpr (DIM == 3) {
+                if (narrow) scan_rows(std::true_type{});
+                else scan_rows(std::false_type{});
+            } else {
+                scan_rows(std::false_type{});
             }
             flush();
             __syncthreads();
