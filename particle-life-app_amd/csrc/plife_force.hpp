@@ -279,6 +279,7 @@ __device__ void home_global(const ForceArgs &a, const WorldDev &w, const double 
 // when the neighbourhood is denser than the staging buffer.  Workgroups with
 // more parts or staged cells than the cell table holds take the global path.
 constexpr int kMaxParts = 4;
+constexpr int kScreenPad = 4;    // >= the screen's group size U
 constexpr int kDealKeys = 5;     // 4 position groups + the inactive threads of a short block
 
 // Position group of a home inside its cell: y quartile (2D) or y/z halves (3D).
@@ -303,12 +304,14 @@ __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int nt
     ForceLds L;
     size_t o = 0;
     L.m = o;     o += m_in_lds ? (size_t)ntypes * ntypes * 8 : 0;
-    L.sxy = o;   o += (size_t)cap * 8;
-    L.sq = o;    o += dim == 3 ? (size_t)cap * 4 : 0;     // 16-bit screen (3D only)
+    // the screened arrays carry kScreenPad spare entries: a screen step reads
+    // a group of U candidates from one base, up to U - 1 past a lane's range
+    L.sxy = o;   o += (size_t)(cap + kScreenPad) * 8;
+    L.sq = o;    o += dim == 3 ? (size_t)(cap + kScreenPad) * 4 : 0;     // 16-bit screen (3D only)
     L.lists = o; o += (size_t)lcap * bs * 2;
     o = (o + 3) & ~(size_t)3;
-    L.sz = o;    o += dim == 3 ? (size_t)cap * 4 : 0;
-    L.sqz = o;   o += dim == 3 ? (size_t)cap * 2 : 0;
+    L.sz = o;    o += dim == 3 ? (size_t)(cap + kScreenPad) * 4 : 0;
+    L.sqz = o;   o += dim == 3 ? (size_t)(cap + kScreenPad) * 2 : 0;
     o = (o + 3) & ~(size_t)3;
     L.sid = o;   o += debug ? (size_t)cap * 4 : 0;
     L.off = o;   o += (size_t)(max_cells + 1) * 4;
@@ -565,10 +568,14 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             }
             __syncthreads();
 
-            int cnt = 0;
+            // Per-lane candidate list: entries at lists[j * BS + tid]; lp walks
+            // the lane's column of the list in steps of BS entries.
+            uint16_t *const lbase = lists + tid;
+            uint16_t *lp = lbase;
+            uint16_t *const lfull = lbase + (a.lcap - U) * BS;     // flush beyond this
             auto flush = [&]() {
-                for (int j = 0; j < cnt; ++j) {
-                    const int k = lists[j * BS + tid];
+                for (uint16_t *e = lbase; e < lp; e += BS) {
+                    const int k = *e;
                     const uint2 q = sxy[k];
                     const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
                     const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
@@ -583,7 +590,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
                     pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
                 }
-                cnt = 0;
+                lp = lbase;
             };
             // the row scan, instantiated for the 16-bit and the 32-bit screen
             auto scan_rows = [&](auto narrow_tag) {
@@ -598,25 +605,32 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     const int len = skip ? 0 : max(0, khi - klo);
                     const int K = wave_max(len);
                     for (int i = 0; i < K; i += U) {
-                        if (__any(cnt > a.lcap - U)) flush();
+                        if (__any(lp > lfull)) flush();
+                        // U candidates from one base: a lane past its range reads the
+                        // window's first entries instead (and keeps none of them)
+                        const int kb = i < len ? klo + i : 0;
+                        bool pass[U];
+                        if constexpr (NARROW) {
+                            const uint32_t *qp = sq + kb;
+                            const uint16_t *qzp = sqz + kb;
+#pragma unroll
+                            for (int u = 0; u < U; ++u)
+                                pass[u] = screen16<DIM>(qp[u], hq, DIM == 3 ? (uint32_t)qzp[u] : 0u, hqz, bq);
+                        } else {
+                            const uint2 *qp = sxy + kb;
+                            const uint32_t *qzp = szz + kb;
+#pragma unroll
+                            for (int u = 0; u < U; ++u)
+                                pass[u] = screen<DIM, WRAP>(hx, hy, hz, qp[u].x, qp[u].y,
+                                                            DIM == 3 ? qzp[u] : 0u, w.r2hi);
+                        }
+                        // branch-free append: every candidate is written to the next
+                        // free slot, which only a kept one claims (the flush check
+                        // above leaves room for U slots)
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
-                            const int ii = i + u;
-                            if (ii < len) {
-                                const int k = klo + ii;
-                                bool pass;
-                                if constexpr (NARROW) {
-                                    pass = screen16<DIM>(sq[k], hq, DIM == 3 ? sqz[k] : 0u, hqz, bq);
-                                } else {
-                                    const uint2 q = sxy[k];
-                                    pass = screen<DIM, WRAP>(hx, hy, hz, q.x, q.y,
-                                                             DIM == 3 ? szz[k] : 0u, w.r2hi);
-                                }
-                                if (pass) {
-                                    lists[cnt * BS + tid] = (uint16_t)k;
-                                    ++cnt;
-                                }
-                            }
+                            *lp = (uint16_t)(kb + u);
+                            lp += (pass[u] && i + u < len) ? BS : 0;
                         }
                     }
                 }
