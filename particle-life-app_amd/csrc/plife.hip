@@ -705,7 +705,7 @@ ForcePlan plan_force(const plife_physics *h)
     const plife_tuning &t = h->tune;
     ForcePlan p;
     p.bs = t.block_homes ? t.block_homes : (clustered ? 128 : 256);
-    p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))
+    p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))   // >= the screen group (8)
                             : (clustered || occ < 12.0 ? 32 : 24);
     p.lcap = std::min(p.lcap, 16384 / p.bs);             // lists stay within 32 KB of LDS
     // a workgroup of bs homes spans ~bs/avg + 1 cells of a row; it stages NR

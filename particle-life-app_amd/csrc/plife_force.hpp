@@ -279,7 +279,7 @@ __device__ void home_global(const ForceArgs &a, const WorldDev &w, const double 
 // when the neighbourhood is denser than the staging buffer.  Workgroups with
 // more parts or staged cells than the cell table holds take the global path.
 constexpr int kMaxParts = 4;
-constexpr int kScreenPad = 4;    // >= the screen's group size U
+constexpr int kScreenPad = 8;    // >= the screen's group size U
 constexpr int kDealKeys = 5;     // 4 position groups + the inactive threads of a short block
 
 // Position group of a home inside its cell: y quartile (2D) or y/z halves (3D).
@@ -328,7 +328,7 @@ template <int DIM, bool WRAP, bool DEBUG, bool MLDS, int BS>
 __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
 {
     constexpr int NR = DIM == 3 ? 9 : 3;      // stencil rows per part
-    constexpr int U = 4;                      // screen unroll; lists keep U slots free
+    constexpr int U = 8;                      // screen group; lists keep U slots free (lcap >= U)
     constexpr int NW = BS / kWave;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_part[kMaxParts][3];      // row, first column, last column
