@@ -138,7 +138,10 @@ int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint
  * candidate-list entries of the force kernel; stage_slack in [0, 16]: LDS
  * staging capacity as a multiple of the expected staged particles;
  * direct_launch 1: launch every kernel directly instead of replaying a HIP
- * graph of two steps.  Results do not depend on the tuning.  Automatic
+ * graph of two steps; sparse_worlds: which worlds the sparse-world force
+ * kernel (one thread per particle, stencil read from global memory) steps
+ * instead of the staged one -- 0 those whose cells hold too few particles
+ * for staging to pay, 1 none, 2 all.  Results do not depend on the tuning.  Automatic
  * fields follow the world's clustering: every step measures the mean size of
  * the cell a particle sits in, and each plife_update plans its launches from
  * the latest measurement that has reached the host. */
@@ -147,6 +150,7 @@ typedef struct {
     int32_t list_cap;
     int32_t direct_launch;
     double stage_slack;
+    int32_t sparse_worlds;
 } plife_tuning;
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t);

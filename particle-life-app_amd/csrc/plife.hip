@@ -95,17 +95,44 @@ void dev_free(T *&p)
     p = nullptr;
 }
 
-// Grow-only device array: reallocated (contents dropped) when too small.
+// Capacity for `need` elements when an array of capacity `cap` is too small:
+// exact the first time, then at least 1.5x, so that a batch growing one world
+// at a time reallocates O(log n) times (hipFree waits for the whole device,
+// including other handles' streams).
+inline size_t grown(size_t cap, size_t need) { return cap ? std::max(need, cap + cap / 2) : need; }
+
+// Arrays of at most this size that a larger one replaces are kept until the
+// handle is destroyed instead of freed at once: hipFree waits for the whole
+// device, and with 1.5x growth the kept arrays total less than twice the
+// live one.
+constexpr size_t kRetireMax = (size_t)32 << 20;
+
+// One pinned staging buffer of table uploads and the event recorded after the
+// copies that read it.
+struct UploadSlot {
+    unsigned char *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+constexpr int kUploadSlots = 4;
+
+// Grow-only device array: reallocated (contents dropped) when too small.  The
+// old array goes to `retired` when given and small enough, else is freed.
 template <typename T>
-int ensure_dev(T **p, size_t *capv, size_t need)
+int ensure_dev(T **p, size_t *capv, size_t need, std::vector<void *> *retired = nullptr)
 {
     need = std::max<size_t>(need, 1);
     if (need <= *capv && *p) return PLIFE_OK;
+    const size_t cap = grown(*p ? *capv : 0, need);
+    if (*p && retired && *capv * sizeof(T) <= kRetireMax) {
+        retired->push_back((void *)*p);
+        *p = nullptr;
+    }
     dev_free(*p);
     *capv = 0;
-    int rc = dev_alloc(p, need);
+    int rc = dev_alloc(p, cap);
     if (rc) return rc;
-    *capv = need;
+    *capv = cap;
     return PLIFE_OK;
 }
 
@@ -135,6 +162,8 @@ struct WorldHost {
     int n = 0, off = 0;
     int64_t steps = 0;
     bool live = true;                  // false once removed (dropped at the next layout)
+    double nb = -1.0;                  // neighbours per particle in the last measured step
+    bool sparse = false;               // stepped by the sparse-world force kernel
 };
 
 struct plife_physics {
@@ -169,8 +198,10 @@ struct plife_physics {
     std::vector<uint2> blocks;
     uint2 *d_blocks = nullptr;
     size_t blocks_cap = 0;
-    int blocks_bs = 0, nb_wrap = 0, nb_total = 0;
-    uint64_t blocks_version = ~0ull;
+    int blocks_bs = 0, nb_total = 0;
+    int seg[4] = {};                   // blocks of (periodic, clamped) x (staged, sparse) worlds
+    uint64_t blocks_version = ~0ull, blocks_classes = ~0ull;
+    uint64_t classes = 0;              // bumped whenever a world changes force kernel
 
     // particle state: positions ping-pong P[2], the rest A[2]
     int n = 0, cap = 0;
@@ -187,12 +218,21 @@ struct plife_physics {
     uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_end = nullptr;
     size_t hist_cap = 0, bsum_cap = 0;
     uint32_t *hist = nullptr, *bsum = nullptr;
+    unsigned long long *bsq = nullptr;   // per-block sums of squares of the cell scan
+    size_t bsq_cap = 0;
     // [0] interacting pairs, [1] sum of cell counts^2, [2 + w] pairs of world w
     unsigned long long *d_pairs = nullptr;
     size_t pairs_cap = 0;
-    uint64_t *stat_host = nullptr;     // pinned copy of d_pairs[1], written every step
+    // pinned copy of d_pairs (3 + worlds entries), written by every step
+    uint64_t *stat_host = nullptr;
+    size_t stat_cap = 0;
+    std::vector<void *> retired_host;  // replaced pinned buffers, freed at destroy
     unsigned char *pin = nullptr;      // pinned staging of state downloads
     size_t pin_cap = 0;
+    UploadSlot up[kUploadSlots];       // pinned staging of table uploads, used in turn
+    int up_cur = 0;
+    size_t up_used = 0;
+    std::vector<void *> retired;       // replaced small device arrays, freed at destroy
     double occupancy = 0.0;            // particle-weighted cell occupancy seen by the planner
     bool clustered = false;            // the planner's regime (with hysteresis)
 
@@ -330,8 +370,13 @@ int ensure_capacity(plife_physics *h, int need, bool keep)
 
 int ensure_cells(plife_physics *h, long long ncells)
 {
-    const size_t c = (size_t)std::max(ncells, 1LL);
-    if (c <= h->cells_cap && h->cell_count) return PLIFE_OK;
+    const size_t need = (size_t)std::max(ncells, 1LL);
+    if (need <= h->cells_cap && h->cell_count) return PLIFE_OK;
+    const size_t c = grown(h->cell_count ? h->cells_cap : 0, need);
+    if (h->cell_count && h->cells_cap * 4 <= kRetireMax) {
+        h->retired.insert(h->retired.end(), {h->cell_count, h->cell_start, h->cell_end});
+        h->cell_count = h->cell_start = h->cell_end = nullptr;
+    }
     dev_free(h->cell_count);
     dev_free(h->cell_start);
     dev_free(h->cell_end);
@@ -342,7 +387,6 @@ int ensure_cells(plife_physics *h, long long ncells)
     if ((rc = dev_alloc(&h->cell_end, c))) return rc;
     // the step keeps cell_count all-zero between steps (the scan clears it)
     HIP_TRY(hipMemsetAsync(h->cell_count, 0, c * 4, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
     h->cells_cap = c;
     return PLIFE_OK;
 }
@@ -389,9 +433,53 @@ WorldDev world_dev(const WorldHost &w, uint32_t cell_base, int moff)
     return d;
 }
 
+// Table uploads go through pinned buffers and are ordered on the stream, so
+// the host does not wait for them.  A batch of uploads (begin_uploads with
+// room for `bytes`, upload calls, end_uploads) takes the next of
+// kUploadSlots buffers, waiting only until the batch that used it last has
+// been copied.
+int begin_uploads(plife_physics *h, size_t bytes)
+{
+    h->up_cur = (h->up_cur + 1) % kUploadSlots;
+    UploadSlot &s = h->up[h->up_cur];
+    HIP_TRY(hipEventSynchronize(s.done));
+    h->up_used = 0;
+    if (bytes <= s.cap) return PLIFE_OK;
+    const size_t cap = std::max({bytes, s.cap * 2, (size_t)64 * 1024});
+    if (s.p) (void)hipHostFree(s.p);
+    s.p = nullptr;
+    s.cap = 0;
+    const hipError_t e = hipHostMalloc((void **)&s.p, cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        s.p = nullptr;
+        return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", cap, hipGetErrorString(e));
+    }
+    s.cap = cap;
+    return PLIFE_OK;
+}
+
+int upload(plife_physics *h, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return PLIFE_OK;
+    UploadSlot &s = h->up[h->up_cur];
+    const size_t at = (h->up_used + 255) & ~(size_t)255;
+    if (at + bytes > s.cap) return fail(PLIFE_ERR_STATE, "upload staging too small");
+    std::memcpy(s.p + at, src, bytes);
+    h->up_used = at + bytes;
+    HIP_TRY(hipMemcpyAsync(dst, s.p + at, bytes, hipMemcpyHostToDevice, h->stream));
+    return PLIFE_OK;
+}
+
+int end_uploads(plife_physics *h)
+{
+    HIP_TRY(hipEventRecord(h->up[h->up_cur].done, h->stream));
+    return PLIFE_OK;
+}
+
 // Rebuilds the world table (offsets, cell bases, matrix offsets) from
-// h->worlds and uploads it with the matrices.
-int upload_params(plife_physics *h)
+// h->worlds and uploads it with the matrices, leaving `extra` bytes of the
+// upload staging for the caller's own uploads in the same batch.
+int upload_params(plife_physics *h, size_t extra)
 {
     const int W = (int)h->worlds.size();
     h->wdev.resize(W);
@@ -410,16 +498,28 @@ int upload_params(plife_physics *h)
     h->ncells = cells;
     h->ntypes_max = tmax;
     int rc;
-    if ((rc = ensure_dev(&h->d_worlds, &h->worlds_cap, (size_t)W))) return rc;
-    if ((rc = ensure_dev(&h->d_matrix, &h->matrix_cap, h->matrices.size()))) return rc;
-    if ((rc = ensure_dev(&h->d_pairs, &h->pairs_cap, (size_t)(2 + W)))) return rc;
-    if (W) HIP_TRY(hipMemcpyAsync(h->d_worlds, h->wdev.data(), W * sizeof(WorldDev),
-                                  hipMemcpyHostToDevice, h->stream));
-    if (!h->matrices.empty())
-        HIP_TRY(hipMemcpyAsync(h->d_matrix, h->matrices.data(), h->matrices.size() * sizeof(double),
-                               hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));     // the host copies may change after this
-    return PLIFE_OK;
+    if ((rc = ensure_dev(&h->d_worlds, &h->worlds_cap, (size_t)W, &h->retired))) return rc;
+    if ((rc = ensure_dev(&h->d_matrix, &h->matrix_cap, h->matrices.size(), &h->retired))) return rc;
+    unsigned long long *const old_pairs = h->d_pairs;
+    if ((rc = ensure_dev(&h->d_pairs, &h->pairs_cap, (size_t)(3 + W), &h->retired))) return rc;
+    if (h->d_pairs != old_pairs)        // per-world counters read before the next step
+        HIP_TRY(hipMemsetAsync(h->d_pairs, 0, h->pairs_cap * 8, h->stream));
+    if ((size_t)(3 + W) > h->stat_cap) {
+        // the old buffer may still be written by queued steps: keep it until destroy
+        const size_t cap = grown(h->stat_cap, (size_t)(3 + W));
+        uint64_t *p = nullptr;
+        const hipError_t e = hipHostMalloc((void **)&p, cap * 8, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
+        std::memset(p, 0, cap * 8);
+        p[1] = h->stat_host ? h->stat_host[1] : 0;
+        if (h->stat_host) h->retired_host.push_back(h->stat_host);
+        h->stat_host = p;
+        h->stat_cap = cap;
+    }
+    const size_t wb = W * sizeof(WorldDev), mb = h->matrices.size() * sizeof(double);
+    if ((rc = begin_uploads(h, wb + mb + extra + 1024))) return rc;
+    if ((rc = upload(h, h->d_worlds, h->wdev.data(), wb))) return rc;
+    return upload(h, h->d_matrix, h->matrices.data(), mb);
 }
 
 // Brings the device layout up to date: drops removed worlds (one compaction
@@ -451,21 +551,18 @@ int sync_layout(plife_physics *h)
         for (int w = 0; w < (int)h->worlds.size(); ++w) h->index_of[h->worlds[w].id] = w;
         const int W = (int)h->worlds.size();
         const int nnew = off;
-        if ((rc = upload_params(h))) return rc;
+        if ((rc = upload_params(h, W * sizeof(int)))) return rc;
         if (reindexed)      // per-world pair counters of the last step refer to old indices
-            HIP_TRY(hipMemsetAsync(h->d_pairs, 0, (size_t)(2 + W) * 8, h->stream));
+            HIP_TRY(hipMemsetAsync(h->d_pairs, 0, (size_t)(3 + W) * 8, h->stream));
         if (W > 1 && nnew > 0) {
-            if ((rc = ensure_dev(&h->world_of, &h->world_of_cap, (size_t)nnew))) return rc;
-            for (int w = 0; w < W; ++w)
-                if (h->worlds[w].n > 0)
-                    k_fill_u16<<<ceil_div(h->worlds[w].n, kBlock), kBlock, 0, h->stream>>>(
-                        h->world_of + h->worlds[w].off, (uint16_t)w, h->worlds[w].n);
+            if ((rc = ensure_dev(&h->world_of, &h->world_of_cap, (size_t)nnew, &h->retired))) return rc;
+            k_world_of<<<ceil_div(nnew, kBlock), kBlock, 0, h->stream>>>(nnew, h->d_worlds, W,
+                                                                        h->world_of);
             LAUNCH_CHECK();
         }
         if (moved && nnew > 0) {
-            if ((rc = ensure_dev(&h->d_src_off, &h->src_off_cap, (size_t)W))) return rc;
-            HIP_TRY(hipMemcpyAsync(h->d_src_off, src.data(), W * sizeof(int), hipMemcpyHostToDevice,
-                                   h->stream));
+            if ((rc = ensure_dev(&h->d_src_off, &h->src_off_cap, (size_t)W, &h->retired))) return rc;
+            if ((rc = upload(h, h->d_src_off, src.data(), W * sizeof(int)))) return rc;
             const int p = h->pc, a = h->ac;
             k_compact<<<ceil_div(nnew, kBlock), kBlock, 0, h->stream>>>(
                 nnew, h->d_worlds, W > 1 ? h->world_of : nullptr, h->d_src_off, h->px[p], h->py[p],
@@ -476,12 +573,13 @@ int sync_layout(plife_physics *h)
             h->pc = 1 - p;
             h->ac = 1 - a;
         }
-        HIP_TRY(hipStreamSynchronize(h->stream));
+        if ((rc = end_uploads(h))) return rc;
         h->n = nnew;
         h->used = nnew;
         h->stepped = false;
-    } else if ((rc = upload_params(h))) {
-        return rc;
+    } else {
+        if ((rc = upload_params(h, 0))) return rc;
+        if ((rc = end_uploads(h))) return rc;
     }
     h->layout_dirty = false;
     h->params_dirty = false;
@@ -489,30 +587,38 @@ int sync_layout(plife_physics *h)
     return PLIFE_OK;
 }
 
-// Force workgroup table for block size bs: periodic worlds' blocks, then clamped.
+// Force workgroup table: segments of periodic staged, periodic sparse,
+// clamped staged and clamped sparse worlds (one launch each), bs homes per
+// staged block and kSparseBlock per sparse block.
 int ensure_blocks(plife_physics *h, int bs)
 {
-    if (h->blocks_bs == bs && h->blocks_version == h->version) return PLIFE_OK;
+    if (h->blocks_bs == bs && h->blocks_version == h->version && h->blocks_classes == h->classes)
+        return PLIFE_OK;
     h->blocks.clear();
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int s = 0; s < 4; ++s) {
+        const bool wrap = s < 2, sparse = (s & 1) != 0;
+        const size_t before = h->blocks.size();
         for (int w = 0; w < (int)h->worlds.size(); ++w) {
             const WorldHost &wh = h->worlds[w];
-            if ((wh.st.wrap != 0) != (pass == 0)) continue;
-            for (int g = 0; g < wh.n; g += bs)
+            if ((wh.st.wrap != 0) != wrap || wh.sparse != sparse) continue;
+            const int step = sparse ? kSparseBlock : bs;
+            for (int g = 0; g < wh.n; g += step)
                 h->blocks.push_back(make_uint2((uint32_t)w, (uint32_t)(wh.off + g)));
         }
-        if (pass == 0) h->nb_wrap = (int)h->blocks.size();
+        h->seg[s] = (int)(h->blocks.size() - before);
     }
     h->nb_total = (int)h->blocks.size();
     int rc;
-    if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size()))) return rc;
+    if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size(), &h->retired))) return rc;
     if (!h->blocks.empty()) {
-        HIP_TRY(hipMemcpyAsync(h->d_blocks, h->blocks.data(), h->blocks.size() * sizeof(uint2),
-                               hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
+        const size_t bytes = h->blocks.size() * sizeof(uint2);
+        if ((rc = begin_uploads(h, bytes))) return rc;
+        if ((rc = upload(h, h->d_blocks, h->blocks.data(), bytes))) return rc;
+        if ((rc = end_uploads(h))) return rc;
     }
     h->blocks_bs = bs;
     h->blocks_version = h->version;
+    h->blocks_classes = h->classes;
     return PLIFE_OK;
 }
 
@@ -548,8 +654,11 @@ int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
     const int nb0 = ceil_div(a.n, kScanTile), nb1 = b.n > 0 ? ceil_div(b.n, kScanTile) : 0;
     if ((size_t)(nb0 + nb1) > h->bsum_cap || !h->bsum)
         return fail(PLIFE_ERR_STATE, "scan scratch was not prepared");
+    if ((a.sq && (!h->bsq || (size_t)nb0 > h->bsq_cap)) || b.sq)
+        return fail(PLIFE_ERR_STATE, "scan scratch was not prepared");
     a.bsum = h->bsum;
     b.bsum = h->bsum + nb0;
+    a.sqb = h->bsq;
     k_scan2_reduce<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
     k_scan2_single<<<nb1 ? 2 : 1, kBlock, 0, h->stream>>>(a, b);
     k_scan2_down<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
@@ -615,7 +724,8 @@ void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, int ntiles)
 #define PLIFE_BIN(DD)                                                                          \
     k_bin_tile<DIM, DD><<<ntiles, 1024, 0, h->stream>>>(P.x, P.y, P.z, n, h->d_worlds, wof,    \
                                                           ntiles, h->keys0, h->cell_count,     \
-                                                          h->hist, h->d_pairs, ncounters)
+                                                          h->hist, h->d_pairs, ncounters,      \
+                                                          h->version + 1)
     switch (D) {
         case 1: PLIFE_BIN(1); break;
         case 2: PLIFE_BIN(2); break;
@@ -646,16 +756,37 @@ void launch_force(plife_physics *h, const ForceArgs &a, int nblocks, int bs, siz
     else launch_force_bs<DIM, WRAP, DEBUG, false>(h, a, nblocks, bs, lds);
 }
 
-// One launch per wrap mode present in the batch (periodic worlds' blocks
-// come first in the block table).
+template <int DIM, bool WRAP, bool DEBUG>
+void launch_sparse(plife_physics *h, const ForceArgs &a, int nblocks)
+{
+    const size_t lds = a.m_in_lds ? (size_t)a.ntypes_max * a.ntypes_max * 8 : 0;
+    if (a.m_in_lds) k_force_sparse<DIM, WRAP, DEBUG, true><<<nblocks, kSparseBlock, lds, h->stream>>>(a);
+    else k_force_sparse<DIM, WRAP, DEBUG, false><<<nblocks, kSparseBlock, 0, h->stream>>>(a);
+}
+
+// One launch per segment of the block table present in the batch: periodic
+// staged, periodic sparse, clamped staged, clamped sparse.
 int launch_force_any(plife_physics *h, ForceArgs a, int bs, size_t lds)
 {
     const bool dbg = h->debug != 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool wrap = pass == 0;
-        a.block_base = wrap ? 0 : h->nb_wrap;
-        const int nb = wrap ? h->nb_wrap : h->nb_total - h->nb_wrap;
+    int base = 0;
+    for (int s = 0; s < 4; ++s) {
+        const bool wrap = s < 2, sparse = (s & 1) != 0;
+        const int nb = h->seg[s];
+        a.block_base = base;
+        base += nb;
         if (nb <= 0) continue;
+        if (sparse) {
+            if (h->dim == 3) {
+                if (wrap) dbg ? launch_sparse<3, true, true>(h, a, nb) : launch_sparse<3, true, false>(h, a, nb);
+                else dbg ? launch_sparse<3, false, true>(h, a, nb) : launch_sparse<3, false, false>(h, a, nb);
+            } else {
+                if (wrap) dbg ? launch_sparse<2, true, true>(h, a, nb) : launch_sparse<2, true, false>(h, a, nb);
+                else dbg ? launch_sparse<2, false, true>(h, a, nb) : launch_sparse<2, false, false>(h, a, nb);
+            }
+            LAUNCH_CHECK();
+            continue;
+        }
         if (h->dim == 3) {
             if (wrap) dbg ? launch_force<3, true, true>(h, a, nb, bs, lds) : launch_force<3, true, false>(h, a, nb, bs, lds);
             else dbg ? launch_force<3, false, true>(h, a, nb, bs, lds) : launch_force<3, false, false>(h, a, nb, bs, lds);
@@ -699,7 +830,18 @@ ForcePlan plan_force(const plife_physics *h)
 {
     const int dim = h->dim == 3 ? 3 : 2;
     const int NR = dim == 3 ? 9 : 3;
-    const double avg = (double)h->n / (double)std::max(h->ncells, 1LL);
+    // density of the worlds the staged kernel steps (all of them if none is)
+    long long sn = 0, sc = 0;
+    for (const WorldHost &w : h->worlds)
+        if (!w.sparse) {
+            sn += w.n;
+            sc += grid_of(w.st).ncells;
+        }
+    if (sc == 0) {
+        sn = h->n;
+        sc = h->ncells;
+    }
+    const double avg = (double)sn / (double)std::max(sc, 1LL);
     const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
     const bool clustered = h->clustered;
     const plife_tuning &t = h->tune;
@@ -743,7 +885,16 @@ ForcePlan plan_force(const plife_physics *h)
 // (a pinned copy refreshed by every step; a stale value only affects speed).
 void refresh_occupancy(plife_physics *h)
 {
-    const uint64_t q = __atomic_load_n(h->stat_host, __ATOMIC_RELAXED);
+    const int W = (int)h->worlds.size();
+    // per-world neighbour counts, when the copy comes from the current layout
+    if (h->stat_host && (size_t)(3 + W) <= h->stat_cap &&
+        __atomic_load_n(h->stat_host + 2 + W, __ATOMIC_RELAXED) == h->version + 1)
+        for (int w = 0; w < W; ++w) {
+            const uint64_t p = __atomic_load_n(h->stat_host + 2 + w, __ATOMIC_RELAXED);
+            if (h->worlds[w].n > 0 && h->worlds[w].steps > 0)
+                h->worlds[w].nb = (double)p / (double)h->worlds[w].n;
+        }
+    const uint64_t q = h->stat_host ? __atomic_load_n(h->stat_host + 1, __ATOMIC_RELAXED) : 0;
     if (q == 0 || h->n <= 0 || h->ncells <= 0) return;
     h->occupancy = (double)q / (double)h->n;
     const double avg = (double)h->n / (double)h->ncells;
@@ -754,6 +905,49 @@ void refresh_occupancy(plife_physics *h)
     else if (h->clustered && (ratio < 3.0 || h->occupancy < 48.0)) h->clustered = false;
 }
 
+// Which force kernel steps each world.  A world goes to the sparse-world
+// kernel when a staged workgroup of its homes would span more than two grid
+// rows or more cells than the staged cell table holds; the span follows from
+// the occupancy of the cells its particles sit in, estimated from the
+// neighbours per particle of the last measured step (a uniform world's count
+// before that): local density nb / V(rmax) times the cell volume rmax^dim.
+// Grids of fewer than 3 cells per axis always take it (literal stencil over
+// repeated cells).  Hysteresis keeps worlds near the threshold from
+// switching kernels, and so recapturing, at every update.  Results do not
+// depend on the class.
+void classify_worlds(plife_physics *h)
+{
+    const int dim = h->dim == 3 ? 3 : 2;
+    const int NR = dim == 3 ? 9 : 3;
+    const int mode = h->tune.sparse_worlds;
+    const double bs = h->tune.block_homes ? h->tune.block_homes : (h->clustered ? 128 : 256);
+    bool changed = false;
+    for (WorldHost &w : h->worlds) {
+        const Grid gr = grid_of(w.st);
+        bool sparse;
+        if (gr.nx < 3) {
+            sparse = true;
+        } else if (mode != 0) {
+            sparse = mode == 2;
+        } else {
+            const double r = w.st.rmax;
+            const double cell = dim == 3 ? r * r * r : r * r;
+            const double ball = dim == 3 ? 4.0 / 3.0 * M_PI * r * r * r : M_PI * r * r;
+            const double nb = w.nb >= 0.0 ? w.nb : std::max(w.n - 1, 0) * ball;
+            const double avg = (double)w.n / (double)gr.ncells;
+            const double occ = std::max({avg, nb * cell / ball, 1e-9});
+            const double span = bs / occ;              // cells spanned by a workgroup's homes
+            const double k = w.sparse ? 0.75 : 1.0;
+            sparse = span > 2.0 * gr.nx * k || NR * (span + 6.0) > 2048.0 * k;
+        }
+        if (sparse != w.sparse) {
+            w.sparse = sparse;
+            changed = true;
+        }
+    }
+    if (changed) ++h->classes;
+}
+
 // Everything a step of the current configuration needs, so that the steps
 // (captured or direct) upload and allocate nothing.
 int prepare_step(plife_physics *h)
@@ -761,14 +955,17 @@ int prepare_step(plife_physics *h)
     int rc;
     if ((rc = sync_layout(h))) return rc;
     refresh_occupancy(h);
+    classify_worlds(h);
     h->plan = plan_force(h);
     if ((rc = ensure_cells(h, h->ncells))) return rc;
     if ((rc = ensure_debug(h))) return rc;
     const RadixPlan rp = radix_plan(h->ncells, h->n);
-    if ((rc = ensure_dev(&h->hist, &h->hist_cap, rp.hist_len))) return rc;
+    if ((rc = ensure_dev(&h->hist, &h->hist_cap, rp.hist_len, &h->retired))) return rc;
     const size_t nb = (size_t)ceil_div(h->ncells, kScanTile) +
                       (size_t)ceil_div((long long)rp.hist_len, kScanTile);
-    if ((rc = ensure_dev(&h->bsum, &h->bsum_cap, nb))) return rc;
+    if ((rc = ensure_dev(&h->bsum, &h->bsum_cap, nb, &h->retired))) return rc;
+    if ((rc = ensure_dev(&h->bsq, &h->bsq_cap, (size_t)ceil_div(h->ncells, kScanTile), &h->retired)))
+        return rc;
     return ensure_blocks(h, h->plan.bs);
 }
 
@@ -778,7 +975,8 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     const int n = h->n;
     const int nc = (int)h->ncells;
     int rc;
-    if (h->blocks_bs != h->plan.bs || h->blocks_version != h->version || h->layout_dirty ||
+    if (h->blocks_bs != h->plan.bs || h->blocks_version != h->version ||
+        h->blocks_classes != h->classes || h->layout_dirty ||
         h->params_dirty)
         return fail(PLIFE_ERR_STATE, "the step was not prepared");
 
@@ -807,9 +1005,6 @@ int do_step(plife_physics *h, PhaseEvents *ev)
                         n > 0 ? ScanJob{h->hist, (int)plan.hist_len, h->hist, nullptr, 0, nullptr, nullptr}
                               : no_job())))
         return rc;
-    // the clustering measure travels to pinned host memory for the launch planner
-    HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs + 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                           h->stream));
     h->cells_dirty = false;
     if ((rc = record(h, ev, 1))) return rc;
 
@@ -870,6 +1065,10 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     h->last_bs = fp.bs;
     h->last_lcap = fp.lcap;
     if ((rc = launch_force_any(h, a, fp.bs, fp.lds))) return rc;
+    // the step's counters (pairs, clustering measure, per-world pairs, layout
+    // tag) travel to pinned host memory for the launch planner
+    HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs, (3 + h->worlds.size()) * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, h->stream));
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
     h->ac = ao;
@@ -884,6 +1083,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
         uint64_t version;
         plife_tuning tune;
         ForcePlan plan;
+        int seg[4];
         const void *ptr[36];
     } k;
     std::memset(&k, 0, sizeof k);
@@ -896,11 +1096,13 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     k.version = h->version;
     k.tune = h->tune;
     k.plan = h->plan;
+    std::memcpy(k.seg, h->seg, sizeof k.seg);
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
                        h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
-                       h->cell_start, h->cell_end, h->hist, h->bsum, h->dbg_count, h->dbg_hash,
-                       h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks};
+                       h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash,
+                       h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks,
+                       h->stat_host};
     std::memcpy(k.ptr, p, sizeof p);
     std::vector<unsigned char> v(sizeof k);
     std::memcpy(v.data(), &k, sizeof k);
@@ -1259,9 +1461,14 @@ int create_handle(int32_t device, plife_physics *h)
     if (rc) return rc;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-    e = hipHostMalloc((void **)&h->stat_host, sizeof(uint64_t), hipHostMallocDefault);
+    for (UploadSlot &s : h->up) {
+        e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    e = hipHostMalloc((void **)&h->stat_host, 4 * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
-    *h->stat_host = 0;
+    std::memset(h->stat_host, 0, 4 * sizeof(uint64_t));
+    h->stat_cap = 4;
     return PLIFE_OK;
 }
 
@@ -1276,6 +1483,7 @@ void destroy_handle(plife_physics *h)
     dev_free(h->cell_end);
     dev_free(h->hist);
     dev_free(h->bsum);
+    dev_free(h->bsq);
     dev_free(h->d_pairs);
     dev_free(h->d_matrix);
     dev_free(h->d_worlds);
@@ -1285,16 +1493,26 @@ void destroy_handle(plife_physics *h)
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
     if (h->stat_host) (void)hipHostFree(h->stat_host);
+    for (void *p : h->retired_host) (void)hipHostFree(p);
+    h->retired_host.clear();
     if (h->pin) (void)hipHostFree(h->pin);
+    for (void *p : h->retired) (void)hipFree(p);
+    h->retired.clear();
+    for (UploadSlot &s : h->up) {
+        if (s.p) (void)hipHostFree(s.p);
+        if (s.done) (void)hipEventDestroy(s.done);
+        s = UploadSlot{};
+    }
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
 // A new world: forget the clustering measure of the old ones.
 void reset_planner(plife_physics *h)
 {
-    *h->stat_host = 0;
+    std::memset(h->stat_host, 0, h->stat_cap * sizeof(uint64_t));
     h->occupancy = 0.0;
     h->clustered = false;
+    for (WorldHost &w : h->worlds) w.nb = -1.0;
 }
 
 int check_single(plife_physics *h)
@@ -1317,6 +1535,7 @@ int check_tuning(const plife_tuning *t)
     if (!t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
         t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
+        t->sparse_worlds < 0 || t->sparse_worlds > 2 ||
         (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
          t->block_homes != 256))
         return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");

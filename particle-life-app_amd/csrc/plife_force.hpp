@@ -224,18 +224,63 @@ __device__ __forceinline__ void finish_home(const ForceArgs &a, const WorldDev &
     }
 }
 
+// Candidates lo..hi-1 of the cell-sorted order for home g (slot), FP64
+// throughout: the body of the reference's per-cell loop (Physics.java:413-424).
+template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
+__device__ __forceinline__ void visit_run(const ForceArgs &a, const WorldDev &w, const double *Mlds,
+                                          int g, uint32_t hx, uint32_t hy, uint32_t hz, int row,
+                                          Acc &acc, int lo, int hi)
+{
+    for (int j = lo; j < hi; ++j) {
+        if (j == g) continue;
+        const uint32_t jo = a.perm[j];
+        const uint32_t qx = a.sx[jo], qy = a.sy[jo], qz = DIM == 3 ? a.sz[jo] : 0u;
+        const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
+        const int dzi = DIM == 3 ? (int)(qz - hz) : 0;
+        if (!WRAP) {
+            // plain difference of the clamped world: skip |D| >= 1/2
+            bool ok = ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
+            if (DIM == 3) ok &= ((dzi >= 0) == (qz >= hz));
+            if (!ok) continue;
+        }
+        const int idx = row + a.type[jo];
+        const double m = MLDS ? Mlds[idx] : a.matrix[w.moff + idx];
+        pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, w);
+    }
+}
+
 // Literal reference stencil over 3^DIM cells straight from global memory
 // (Physics.java:404-424), FP64 throughout.  Used for strips whose staged
-// neighbourhood does not fit in LDS and for grids with fewer than 3 cells per
-// axis (where the reference visits repeated cells).
+// neighbourhood does not fit in LDS, for sparse worlds and for grids with
+// fewer than 3 cells per axis (where the reference visits repeated cells).
+// Where a stencil row does not wrap in x its three cells are consecutive keys,
+// so the row is one run of the cell-sorted order: the same candidates in the
+// same order as the per-cell loop, from two bound loads instead of six, all
+// issued before the first candidate is visited.
 template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
 __device__ void home_global(const ForceArgs &a, const WorldDev &w, const double *Mlds, int g,
                             uint32_t hx, uint32_t hy, uint32_t hz, int row, Acc &acc)
 {
+    constexpr int NR = DIM == 3 ? 9 : 3;
     const int nx = w.nx;
     const int cx0 = stencil_origin(hx, w.cs);
     const int cy0 = stencil_origin(hy, w.cs);
     const int cz0 = DIM == 3 ? stencil_origin(hz, w.cs) : 0;
+    if (cx0 >= 1 && cx0 <= nx - 2) {
+        int lo[NR], hi[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int cy = wrap_cell(cy0 + r % 3 - 1, nx);
+            const int cz = DIM == 3 ? wrap_cell(cz0 + r / 3 - 1, nx) : 0;
+            const uint32_t ci = w.cell_base + (uint32_t)(cx0 - 1 + cy * nx + cz * nx * nx);
+            lo[r] = (int)a.cell_start[ci];
+            hi[r] = (int)a.cell_end[ci + 2];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            visit_run<DIM, WRAP, DEBUG, MLDS>(a, w, Mlds, g, hx, hy, hz, row, acc, lo[r], hi[r]);
+        return;
+    }
     const int zlo = DIM == 3 ? -1 : 0, zhi = DIM == 3 ? 1 : 0;
     for (int oz = zlo; oz <= zhi; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
@@ -251,23 +296,8 @@ __device__ void home_global(const ForceArgs &a, const WorldDev &w, const double 
                     continue;
                 }
                 const uint32_t ci = w.cell_base + (uint32_t)(cx + cy * nx + cz * nx * nx);
-                const int lo = (int)a.cell_start[ci], hi = (int)a.cell_end[ci];
-                for (int j = lo; j < hi; ++j) {
-                    if (j == g) continue;
-                    const uint32_t jo = a.perm[j];
-                    const uint32_t qx = a.sx[jo], qy = a.sy[jo], qz = DIM == 3 ? a.sz[jo] : 0u;
-                    const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
-                    const int dzi = DIM == 3 ? (int)(qz - hz) : 0;
-                    if (!WRAP) {
-                        // plain difference of the clamped world: skip |D| >= 1/2
-                        bool ok = ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
-                        if (DIM == 3) ok &= ((dzi >= 0) == (qz >= hz));
-                        if (!ok) continue;
-                    }
-                    const int idx = row + a.type[jo];
-                    const double m = MLDS ? Mlds[idx] : a.matrix[w.moff + idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, w);
-                }
+                visit_run<DIM, WRAP, DEBUG, MLDS>(a, w, Mlds, g, hx, hy, hz, row, acc,
+                                                  (int)a.cell_start[ci], (int)a.cell_end[ci]);
             }
 }
 
@@ -661,6 +691,60 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         unsigned long long sum = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) sum += s_pairs[w];
+        if (sum) {
+            atomicAdd(a.pair_total, sum);
+            atomicAdd(a.pair_world + bw.x, sum);
+        }
+    }
+}
+
+// Worlds whose workgroups would span more grid rows than the staged layout
+// holds (a particle or less per cell): one thread per home of the cell-sorted
+// order, each running home_global, with no LDS but the type matrix, so that
+// many waves per CU hide the latency of the cell bounds and the gathers.
+// Results are those of k_force for the same homes, bit for bit (k_force's
+// global path is this code, and its staged path visits the same candidates in
+// the same order).
+constexpr int kSparseBlock = 256;
+
+template <int DIM, bool WRAP, bool DEBUG, bool MLDS>
+__global__ __launch_bounds__(kSparseBlock) void k_force_sparse(ForceArgs a)
+{
+    constexpr int NW = kSparseBlock / kWave;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned long long s_pairs[NW];
+    double *M = reinterpret_cast<double *>(smem);
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint2 bw = a.blocks[a.block_base + blk];
+    const WorldDev w = a.worlds[bw.x];
+    if constexpr (MLDS) {
+        for (int t = tid; t < w.ntypes * w.ntypes; t += kSparseBlock) M[t] = a.matrix[w.moff + t];
+        __syncthreads();
+    }
+    const int g = (int)bw.y + tid;
+    const bool active = g < w.off + w.n;
+    Acc acc;
+    acc.x = acc.y = acc.z = 0.0;
+    acc.cnt = 0;
+    acc.hash = 0ull;
+    if (active) {
+        const uint32_t ho = a.perm[g];
+        const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
+        const int htype = a.type[ho];
+        home_global<DIM, WRAP, DEBUG, MLDS>(a, w, M, g, hx, hy, hz, htype * w.ntypes, acc);
+        finish_home<DIM, WRAP, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, g);
+    }
+    // interacting pairs, one 64-bit atomic per workgroup
+    unsigned long long tot = active ? (unsigned long long)acc.cnt : 0ull;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) tot += __shfl_xor(tot, d, kWave);
+    if (lane == 0) s_pairs[wv] = tot;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) sum += s_pairs[k];
         if (sum) {
             atomicAdd(a.pair_total, sum);
             atomicAdd(a.pair_world + bw.x, sum);

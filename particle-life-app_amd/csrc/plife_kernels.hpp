@@ -97,12 +97,6 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
     if (i < n) p[i] = v;
 }
 
-__global__ void k_fill_u16(uint16_t *p, uint16_t v, int n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,6 +127,23 @@ struct WorldDev {
     double kdt;                      // force * dt * 2^-32
     double dt;
 };
+
+// World index of every slot of a batch: the last world whose first slot is
+// at or before the slot (worlds are laid out in order; an empty world shares
+// its offset with the next one, so the last such world is the non-empty one).
+__global__ void k_world_of(int n, const WorldDev *__restrict__ worlds, int nworlds,
+                           uint16_t *__restrict__ world_of)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nworlds;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (worlds[mid].off <= i) lo = mid;
+        else hi = mid;
+    }
+    world_of[i] = (uint16_t)lo;
+}
 
 __device__ __forceinline__ int key_axis(uint32_t k, double cs, int nx)
 {
@@ -173,14 +184,16 @@ __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ 
                                                   uint32_t *__restrict__ cell_count,
                                                   uint32_t *__restrict__ hist,
                                                   unsigned long long *__restrict__ counters,
-                                                  int ncounters)
+                                                  int ncounters, unsigned long long tag)
 {
     constexpr int R = 1 << D;
     __shared__ uint32_t h[R];
     for (int t = threadIdx.x; t < R; t += 1024) h[t] = 0;
     // the step's counters (pairs, sum of squares, per-world pairs) start at zero
-    if (blockIdx.x == 0)
+    if (blockIdx.x == 0) {
         for (int t = threadIdx.x; t < ncounters; t += 1024) counters[t] = 0ull;
+        if (threadIdx.x == 0) counters[ncounters] = tag;    // which layout the counts belong to
+    }
     const double cs0 = worlds[0].cs;
     const int nx0 = worlds[0].nx;
     const uint32_t base0 = worlds[0].cell_base;
@@ -223,8 +236,10 @@ constexpr int kScanTile = kBlock * kScanItems;
 // [0, nb0) scan job 0, blocks [nb0, nb0 + nb1) scan job 1.  excl may alias in;
 // with zero_in (requires in != excl) the input is cleared once read, which
 // leaves the cell counters ready for the next step's histogram.  With sq,
-// the reduce pass also adds the sum of squared inputs to *sq (for cell
-// counts: the number of (particle, cell-mate) pairs, a clustering measure).
+// the sum of squared inputs is added to *sq (for cell counts: the number of
+// (particle, cell-mate) pairs, a clustering measure): the reduce pass stores
+// each block's sum in sqb and the block-sum scan adds them up, so the
+// device-wide total costs one atomic rather than one per block.
 struct ScanJob {
     uint32_t *in;
     int n;
@@ -233,6 +248,7 @@ struct ScanJob {
     int zero_in;
     uint32_t *bsum;   // ceil(n / kScanTile) block sums
     unsigned long long *sq;   // may be null
+    unsigned long long *sqb;  // ceil(n / kScanTile) per-block sums of squares (with sq)
 };
 
 __device__ __forceinline__ void scan_job_of(const ScanJob &j0, const ScanJob &j1, int nb0, int b,
@@ -264,16 +280,18 @@ __global__ __launch_bounds__(kBlock) void k_scan2_reduce(ScanJob j0, ScanJob j1,
     if (threadIdx.x == 0) j.bsum[blk] = total;
     if (j.sq) {                                    // uniform per block
         q = block_sum_u64(q, s_q);
-        if (threadIdx.x == 0 && q) atomicAdd(j.sq, q);
+        if (threadIdx.x == 0) j.sqb[blk] = q;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan2_single(ScanJob j0, ScanJob j1)
 {
     __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_q[kBlock / kWave];
     const ScanJob j = blockIdx.x == 0 ? j0 : j1;
     const int n = (j.n + kScanTile - 1) / kScanTile;
     uint32_t carry = 0;
+    unsigned long long q = 0;
     for (int base = 0; base < n; base += kScanTile) {
         const int i0 = base + threadIdx.x * kScanItems;
         uint32_t v[kScanItems];
@@ -282,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_scan2_single(ScanJob j0, ScanJob j1)
         for (int k = 0; k < kScanItems; ++k) {
             v[k] = (i0 + k < n) ? j.bsum[i0 + k] : 0u;
             s += v[k];
+            if (j.sq && i0 + k < n) q += j.sqb[i0 + k];
         }
         uint32_t total;
         uint32_t off = block_excl_scan(s, s_w, total) + carry;
@@ -291,6 +310,10 @@ __global__ __launch_bounds__(kBlock) void k_scan2_single(ScanJob j0, ScanJob j1)
             off += v[k];
         }
         carry += total;
+    }
+    if (j.sq) {                                    // uniform per block
+        q = block_sum_u64(q, s_q);
+        if (threadIdx.x == 0 && q) atomicAdd(j.sq, q);
     }
 }
 

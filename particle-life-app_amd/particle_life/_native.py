@@ -90,6 +90,7 @@ class PlifeTuning(ctypes.Structure):
         ("list_cap", ctypes.c_int32),
         ("direct_launch", ctypes.c_int32),
         ("stage_slack", ctypes.c_double),
+        ("sparse_worlds", ctypes.c_int32),
     ]
 
 

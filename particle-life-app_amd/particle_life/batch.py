@@ -130,9 +130,10 @@ class Batch:
 
     # tuning and timing
     def set_tuning(self, list_cap: int = 0, stage_slack: float = 0.0,
-                   direct_launch: bool = False, block_homes: int = 0) -> None:
+                   direct_launch: bool = False, block_homes: int = 0,
+                   sparse_worlds: int = 0) -> None:
         t = PlifeTuning(int(block_homes), int(list_cap), 1 if direct_launch else 0,
-                        float(stage_slack))
+                        float(stage_slack), int(sparse_worlds))
         check(self._lib.plife_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -140,7 +141,8 @@ class Batch:
         lds = ctypes.c_int32()
         check(self._lib.plife_batch_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lds)))
         return {"block_homes": t.block_homes, "list_cap": t.list_cap, "stage_slack": t.stage_slack,
-                "direct_launch": bool(t.direct_launch), "lds_bytes": lds.value}
+                "direct_launch": bool(t.direct_launch), "sparse_worlds": t.sparse_worlds,
+                "lds_bytes": lds.value}
 
     def time_steps(self, nsteps: int) -> float:
         ms = ctypes.c_double()

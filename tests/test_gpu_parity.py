@@ -147,7 +147,8 @@ def test_launch_plans_bit_identical(eng):
                                    sigma=0.03, dim=2)
     plans = [dict(), dict(block_homes=64, list_cap=8, stage_slack=0.05),
              dict(block_homes=128, list_cap=64, stage_slack=4.0),
-             dict(block_homes=256, list_cap=16, stage_slack=0.3, direct_launch=True)]
+             dict(block_homes=256, list_cap=16, stage_slack=0.3, direct_launch=True),
+             dict(sparse_worlds=2), dict(sparse_worlds=1, block_homes=64)]
     ref = None
     for p in plans:
         eng.set_tuning(**p)
@@ -162,6 +163,46 @@ def test_launch_plans_bit_identical(eng):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=f"plan {p}")
         assert got[3] == ref[3]
     eng.set_tuning()
+
+
+SPARSE = {
+    # name: (seed, n, types, rmax, dim, wrap) -- particles per cell
+    "sparse_3d": (130, 30_000, 6, 0.02, 3, True),          # 0.24
+    "sparse_3d_clamped": (131, 20_000, 5, 0.03, 3, False),  # 0.55
+    "sparse_2d": (132, 3_000, 4, 0.01, 2, True),           # 0.30
+    "sparse_2d_clamped": (133, 2_000, 4, 0.015, 2, False),  # 0.46
+}
+
+
+@pytest.mark.parametrize("name", list(SPARSE))
+def test_sparse_world_kernel(eng, name):
+    """Sparse worlds: the sparse-world kernel (one thread per particle,
+    stencil rows read as runs of the cell-sorted order) matches the oracle,
+    and states after several steps are bit-identical whichever kernel steps
+    the world."""
+    seed, n, t, rmax, dim, wrap = SPARSE[name]
+    print(f"\n[parity] case={name} n={n}", flush=True)
+    w = _world(seed, n, t, rmax, 0.01, dim=dim, wrap=wrap)
+    eng.set_tuning(sparse_worlds=2)
+    try:
+        eng.load_world(w)
+        st = check_one_step(eng, w)
+        print(f"  {st}", flush=True)
+        ref = None
+        for mode in (2, 1, 0):
+            eng.set_tuning(sparse_worlds=mode)
+            eng.load_world(w)
+            eng.step(5)
+            s = eng.state()
+            got = (s["pos"], s["vel"], s["ids"], eng.pair_count())
+            if ref is None:
+                ref = got
+                continue
+            for k in range(3):
+                np.testing.assert_array_equal(got[k], ref[k], err_msg=f"sparse_worlds={mode}")
+            assert got[3] == ref[3]
+    finally:
+        eng.set_tuning()
 
 
 def test_planner_detects_clustering(eng):
