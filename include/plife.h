@@ -98,6 +98,33 @@ int plife_get_count(plife_physics *h, int32_t *n);
  * reference draws from unseeded Math.random; this is its reproducible form. */
 int plife_generate_particles(plife_physics *h, int32_t n, uint64_t seed);
 
+/* Physics.setPositions (Physics.java:166-168) with one of the app's position
+ * presets (app/PositionSetterProvider.java:17-119), numbered in the app's
+ * order: 0 centered, 1 uniform, 2 uniform circle, 3 centered circle, 4 ring,
+ * 5 rainbow ring, 6 color battle, 7 color wheel, 8 line, 9 spiral,
+ * 10 rainbow spiral.  Every particle gets a position from its type, then
+ * wrap or clamp (Physics.ensurePosition), and velocity 0 (Physics.setPosition,
+ * Physics.java:297-303); types are kept.  The reference's setters draw from
+ * unseeded java.util.Random; here particle k of the current order takes
+ * output k of SplitMix64 streams of `seed` (uniform: the generator's 2^-24
+ * grid, as plife_generate_particles; others: 53-bit uniforms and Box-Muller
+ * Gaussians).  3D worlds take z uniform in [0, 1). */
+int plife_set_positions(plife_physics *h, int32_t preset, uint64_t seed);
+
+/* Physics.setTypes (Physics.java:509-511) with one of the app's type presets
+ * (app/TypeSetterProvider.java:12-45), numbered in the app's order: 0 random,
+ * 1 randomize 10%, 2 slices, 3 onion, 4 rotate, 5 flip, 6 more of first,
+ * 7 kill still.  Draws as plife_set_positions. */
+int plife_set_types(plife_physics *h, int32_t preset, uint64_t seed);
+
+/* The app's matrix presets (app/MatrixGeneratorProvider.java:14-79), numbered
+ * in its order: 0 random, 1 symmetry, 2 chains, 3 chains 2, 4 chains 3,
+ * 5 snakes, 6 zero.  Random entries are 2u - 1 (DefaultMatrix.randomize,
+ * DefaultMatrix.java:24-30) with u on the 2^-24 grid of SplitMix64 stream 5
+ * of `seed`, in row order.  Host only (no handle, no device): writes
+ * size * size doubles to out, row = acted-on type, size in [1, 256]. */
+int plife_generate_matrix(int32_t preset, int32_t size, uint64_t seed, double *out);
+
 /* Physics.update() (Physics.java:112-134) applied nsteps times.  Enqueues the
  * steps on the handle's stream and returns; plife_synchronize or a download
  * waits.  Returns PLIFE_STOPPED if plife_force_update_stop() arrived between
@@ -186,6 +213,10 @@ int plife_batch_destroy(plife_batch *b);
 int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
                           const double *matrix, int32_t n, const double *pos, const double *vel,
                           const int32_t *types, int32_t *world_id);
+/* plife_set_positions / plife_set_types for one world of the batch. */
+int plife_batch_set_world_positions(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed);
+int plife_batch_set_world_types(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed);
+
 /* Adds a world whose particles are generated on the device from `seed`, as
  * plife_generate_particles does. */
 int plife_batch_add_generated_world(plife_batch *b, const plife_settings *s, int32_t ntypes,

@@ -1257,20 +1257,6 @@ int profile(plife_physics *h, int nsteps, plife_timing *out)
     return rc;
 }
 
-uint32_t to_fixed(double x, bool wrap)
-{
-    // Physics.ensurePosition (Physics.java:499-505), then round to 2^-32.
-    if (!(x == x)) return 0u;
-    if (wrap) {
-        x = x - std::floor(x);
-        double s = std::rint(x * kFix);
-        return (uint32_t)(unsigned long long)s;
-    }
-    x = x < 0 ? 0 : (x > 1 ? 1 : x);
-    double s = std::rint(x * kFix);
-    return s >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(unsigned long long)s;
-}
-
 int check_settings(const plife_settings *s, int32_t ntypes, const double *matrix)
 {
     if (!s || !matrix) return fail(PLIFE_ERR_ARGUMENT, "null argument");
@@ -1324,9 +1310,9 @@ int upload_particles(plife_physics *h, int off, int n, const double *pos, const 
     std::vector<float> fx(n), fy(n), fz(n);
     std::vector<int32_t> ids(n);
     for (int i = 0; i < n; ++i) {
-        x[i] = to_fixed(pos[3 * i], wrap);
-        y[i] = to_fixed(pos[3 * i + 1], wrap);
-        z[i] = d3 ? to_fixed(pos[3 * i + 2], wrap) : 0u;
+        x[i] = fixed_of(pos[3 * i], wrap);
+        y[i] = fixed_of(pos[3 * i + 1], wrap);
+        z[i] = d3 ? fixed_of(pos[3 * i + 2], wrap) : 0u;
         fx[i] = (float)vel[3 * i];
         fy[i] = (float)vel[3 * i + 1];
         fz[i] = d3 ? (float)vel[3 * i + 2] : 0.0f;
@@ -1370,6 +1356,39 @@ int generate_particles(plife_physics *h, int off, int n, uint64_t seed, int ntyp
     k_gen_uniform<<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(g);
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return PLIFE_OK;
+}
+
+// Applies position preset (positions) or type preset (!positions) `preset`
+// to the slots of world w, with draws from `seed`; the layout must be synced.
+int apply_preset(plife_physics *h, WorldHost &w, bool positions, int preset, uint64_t seed)
+{
+    if (preset < 0 || preset >= (positions ? (int)kPosPresets : (int)kTypePresets))
+        return fail(PLIFE_ERR_ARGUMENT, "%s preset %d out of range", positions ? "position" : "type", preset);
+    if (w.n == 0) return PLIFE_OK;
+    PresetArgs g{};
+    for (int s = 0; s < 6; ++s) g.base[s] = splitmix_mix(seed + (uint64_t)(kPresetStream + s) * kStream);
+    for (int s = 0; s < 4; ++s) g.ubase[s] = splitmix_mix(seed + (uint64_t)(s + 1) * kStream);
+    g.n = w.n;
+    g.off = w.off;
+    g.ntypes = w.ntypes;
+    g.dim = w.st.dim;
+    g.preset = preset;
+    g.wrap = w.st.wrap != 0;
+    const int p = h->pc, a = h->ac;
+    g.px = h->px[p];
+    g.py = h->py[p];
+    g.pz = h->pz[p];
+    g.vx = h->vx[a];
+    g.vy = h->vy[a];
+    g.vz = h->vz[a];
+    g.type = h->type[a];
+    if (positions) k_set_positions<<<ceil_div(w.n, kBlock), kBlock, 0, h->stream>>>(g);
+    else k_set_types<<<ceil_div(w.n, kBlock), kBlock, 0, h->stream>>>(g);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    w.nb = -1.0;             // the world's clustering is no longer known
+    h->stepped = false;      // cell by-products describe the old state
     return PLIFE_OK;
 }
 
@@ -1647,6 +1666,70 @@ int plife_generate_particles(plife_physics *h, int32_t n, uint64_t seed)
     return PLIFE_OK;
 }
 
+int plife_set_positions(plife_physics *h, int32_t preset, uint64_t seed)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = set_device(h))) return rc;
+    if ((rc = sync_layout(h))) return rc;
+    if ((rc = apply_preset(h, h->worlds[0], true, preset, seed))) return rc;
+    reset_planner(h);
+    return PLIFE_OK;
+}
+
+int plife_set_types(plife_physics *h, int32_t preset, uint64_t seed)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = set_device(h))) return rc;
+    if ((rc = sync_layout(h))) return rc;
+    return apply_preset(h, h->worlds[0], false, preset, seed);
+}
+
+int plife_generate_matrix(int32_t preset, int32_t size, uint64_t seed, double *out)
+{
+    if (!out) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (size < 1 || size > kMaxTypes)
+        return fail(PLIFE_ERR_ARGUMENT, "size must be in [1, %d], got %d", kMaxTypes, size);
+    if (preset < 0 || preset > 6) return fail(PLIFE_ERR_ARGUMENT, "matrix preset %d out of range", preset);
+    const int n = size;
+    auto at = [&](int i, int j) -> double & { return out[(size_t)i * n + j]; };
+    const bool random = preset == 0 || preset == 1;
+    // DefaultMatrix.randomize (DefaultMatrix.java:24-30): 2u - 1 for every entry in row order
+    const uint64_t base = splitmix_mix(seed + 5 * kStream);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const uint64_t k = (uint64_t)i * n + j;
+            at(i, j) = random ? 2.0 * ((double)(splitmix_mix(base + (k + 1) * kGamma) >> 40) * 0x1p-24) - 1.0
+                              : 0.0;
+        }
+    switch (preset) {
+        case 1:     // symmetry: the upper triangle copies the lower one
+            for (int i = 0; i < n; ++i)
+                for (int j = i; j < n; ++j) at(i, j) = at(j, i);
+            break;
+        case 2:     // chains: 1 on and beside the diagonal (cyclically), -1 elsewhere
+        case 3:     // chains 2: 1 on the diagonal, 0.2 beside it, -1 elsewhere
+        case 4:     // chains 3: as chains 2 with 0 elsewhere
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    const bool next = j == (i + 1) % n || j == (i + n - 1) % n;
+                    if (preset == 2) at(i, j) = (j == i || next) ? 1.0 : -1.0;
+                    else at(i, j) = j == i ? 1.0 : (next ? 0.2 : (preset == 3 ? -1.0 : 0.0));
+                }
+            break;
+        case 5:     // snakes: 1 on the diagonal, then 0.2 on the next column (cyclically)
+            for (int i = 0; i < n; ++i) {
+                at(i, i) = 1.0;
+                at(i, (i + 1) % n) = 0.2;
+            }
+            break;
+        default:    // random, zero
+            break;
+    }
+    return PLIFE_OK;
+}
+
 int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
                         int32_t *ids)
 {
@@ -1845,6 +1928,26 @@ int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntype
     b->layout_dirty = true;
     *world_id = b->worlds.back().id;
     return PLIFE_OK;
+}
+
+int plife_batch_set_world_positions(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!find_world(b, world_id)) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    int rc = set_device(b);
+    if (rc) return rc;
+    if ((rc = sync_layout(b))) return rc;
+    return apply_preset(b, *find_world(b, world_id), true, preset, seed);
+}
+
+int plife_batch_set_world_types(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!find_world(b, world_id)) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    int rc = set_device(b);
+    if (rc) return rc;
+    if ((rc = sync_layout(b))) return rc;
+    return apply_preset(b, *find_world(b, world_id), false, preset, seed);
 }
 
 int plife_batch_add_generated_world(plife_batch *b, const plife_settings *s, int32_t ntypes,

@@ -445,6 +445,217 @@ __global__ __launch_bounds__(kBlock) void k_gen_uniform(GenArgs g)
     g.id[i] = k;
 }
 
+// Physics.ensurePosition (Physics.java:499-505) of an FP64 coordinate, then
+// rounding to the 2^-32 grid.  Periodic: the fractional part (a result of
+// 1.0, reference quirk Q3, is the same torus point as 0).  Clamped: [0, 1],
+// with 1.0 stored as 1 - 2^-32.  NaN becomes 0.
+__host__ __device__ inline uint32_t fixed_of(double x, bool wrap)
+{
+    if (!(x == x)) return 0u;
+    if (wrap) {
+        x = x - floor(x);
+        return (uint32_t)(unsigned long long)rint(x * kFix);
+    }
+    x = x < 0 ? 0 : (x > 1 ? 1 : x);
+    const double s = rint(x * kFix);
+    return s >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(unsigned long long)s;
+}
+
+// ---------------------------------------------------------------------------
+// The app's position and type presets (app/PositionSetterProvider.java:17-119,
+// app/TypeSetterProvider.java:12-45) over a range of slots, with the
+// counter-based SplitMix64 draws above in place of the reference's unseeded
+// java.util.Random: uniform u = (draw >> 11) * 2^-53, Gaussians by Box-Muller
+// from two streams, sqrt(-2 ln(1 - u1)) cos(2 pi u2).  Shapes are computed in
+// the reference's side-2 frame with its float constants and evaluation order,
+// mapped to the unit box by x * 0.5 + 0.5 as each preset does, then placed
+// by fixed_of.  The uniform preset is the generator above (2^-24 grid,
+// streams 1-3).  3D worlds (no reference counterpart) take z from stream 3.
+// ---------------------------------------------------------------------------
+enum PositionPreset {
+    kPosCentered, kPosUniform, kPosUniformCircle, kPosCenteredCircle, kPosRing, kPosRainbowRing,
+    kPosColorBattle, kPosColorWheel, kPosLine, kPosSpiral, kPosRainbowSpiral, kPosPresets
+};
+enum TypePreset {
+    kTypeRandom, kTypeRandomize10, kTypeSlices, kTypeOnion, kTypeRotate, kTypeFlip,
+    kTypeMoreOfFirst, kTypeKillStill, kTypePresets
+};
+constexpr int kPresetStream = 40;      // preset draws use streams 40..43 (positions), 44..45 (types)
+constexpr double kPi = 3.141592653589793;     // java.lang.Math.PI
+
+struct PresetArgs {
+    uint64_t base[6];                  // streams 40..45
+    uint64_t ubase[4];                 // streams 1..4 (the uniform generator's)
+    int n, off, ntypes, dim, preset, wrap;
+    uint32_t *px, *py, *pz;
+    float *vx, *vy, *vz;
+    int32_t *type;
+};
+
+__device__ __forceinline__ double preset_u(uint64_t base, uint64_t step)
+{
+    return (double)(splitmix_mix(base + step) >> 11) * 0x1p-53;
+}
+
+__device__ __forceinline__ double preset_gauss(uint64_t b1, uint64_t b2, uint64_t step)
+{
+#pragma clang fp contract(off)
+    const double u1 = ((double)(splitmix_mix(b1 + step) >> 11) + 1.0) * 0x1p-53;   // (0, 1]
+    const double u2 = (double)(splitmix_mix(b2 + step) >> 11) * 0x1p-53;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+}
+
+// mapType of TypeSetterProvider.java:46-52
+__device__ __forceinline__ int preset_map_type(double v, int nt)
+{
+    return max(0, min(nt - 1, (int)floor(v * nt)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_set_positions(PresetArgs g)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= g.n) return;
+    const int i = g.off + k;
+    const uint64_t s = (uint64_t)(k + 1) * kGamma;
+    const int t = g.type[i];
+    const double T = (double)g.ntypes;
+    double x = 0.0, y = 0.0;
+    switch (g.preset) {
+        case kPosCentered: {
+            const double scale = (double)0.3f;
+            x = preset_gauss(g.base[0], g.base[1], s) * scale;
+            y = preset_gauss(g.base[2], g.base[3], s) * scale;
+            break;
+        }
+        case kPosUniformCircle: {
+            const double angle = preset_u(g.base[0], s) * 2 * kPi;
+            const double radius = 0.5 * sqrt(preset_u(g.base[1], s));
+            x = cos(angle) * radius;
+            y = sin(angle) * radius;
+            break;
+        }
+        case kPosCenteredCircle: {
+            const double angle = preset_u(g.base[0], s) * 2 * kPi;
+            const double radius = (double)0.5f * preset_u(g.base[1], s);
+            x = cos(angle) * radius;
+            y = sin(angle) * radius;
+            break;
+        }
+        case kPosRing: {
+            const double angle = preset_u(g.base[0], s) * 2 * kPi;
+            const double radius = 0.7 + 0.02 * preset_gauss(g.base[2], g.base[3], s);
+            x = cos(angle) * radius;
+            y = sin(angle) * radius;
+            break;
+        }
+        case kPosRainbowRing: {
+            const double angle = (0.3 * preset_gauss(g.base[0], g.base[1], s) + t) / T * 2 * kPi;
+            const double radius = 0.7 + 0.02 * preset_gauss(g.base[2], g.base[3], s);
+            x = cos(angle) * radius;
+            y = sin(angle) * radius;
+            break;
+        }
+        case kPosColorBattle: {
+            const double ca = t / T * 2 * kPi;
+            const double cr = (double)0.5f;
+            const double angle = preset_u(g.base[0], s) * 2 * kPi;
+            const double radius = preset_u(g.base[1], s) * (double)0.1f;
+            x = cr * cos(ca) + cos(angle) * radius;
+            y = cr * sin(ca) + sin(angle) * radius;
+            break;
+        }
+        case kPosColorWheel: {
+            const double ca = t / T * 2 * kPi;
+            const double cr = (double)0.3f, ir = (double)0.2f;
+            x = cr * cos(ca) + preset_gauss(g.base[0], g.base[1], s) * ir;
+            y = cr * sin(ca) + preset_gauss(g.base[2], g.base[3], s) * ir;
+            break;
+        }
+        case kPosLine:
+            x = 2 * preset_u(g.base[0], s) - 1;
+            y = (2 * preset_u(g.base[1], s) - 1) * (double)0.15f;
+            break;
+        case kPosSpiral:
+        case kPosRainbowSpiral: {
+            double f;
+            if (g.preset == kPosSpiral) {
+                f = preset_u(g.base[0], s);
+            } else {
+                const double spread_t = 0.3 / g.ntypes;
+                f = (t + 1) / (double)(g.ntypes + 2) + spread_t * preset_gauss(g.base[0], g.base[1], s);
+                f = f < 0 ? 0 : (f > 1 ? 1 : f);
+            }
+            const double angle = 2.0 * 2 * kPi * f;
+            const double spread = 0.5 * fmin(f, 0.2);
+            const double radius = 0.9 * f + spread * preset_gauss(g.base[2], g.base[3], s) * spread;
+            x = radius * cos(angle);
+            y = radius * sin(angle);
+            break;
+        }
+        default:    // kPosUniform: the uniform generator's grid values
+            x = (double)(splitmix_mix(g.ubase[0] + s) >> 40) * 0x1p-24;
+            y = (double)(splitmix_mix(g.ubase[1] + s) >> 40) * 0x1p-24;
+            break;
+    }
+    if (g.preset != kPosUniform) {
+        x = x * 0.5 + 0.5;
+        y = y * 0.5 + 0.5;
+    }
+    const bool wrap = g.wrap != 0;
+    g.px[i] = fixed_of(x, wrap);
+    g.py[i] = fixed_of(y, wrap);
+    g.pz[i] = g.dim == 3 ? (uint32_t)(splitmix_mix(g.ubase[2] + s) >> 40) << 8 : 0u;
+    g.vx[i] = 0.0f;     // Physics.setPosition (Physics.java:297-303)
+    g.vy[i] = 0.0f;
+    g.vz[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_set_types(PresetArgs g)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= g.n) return;
+    const int i = g.off + k;
+    const uint64_t s = (uint64_t)(k + 1) * kGamma;
+    const int nt = g.ntypes;
+    int t = g.type[i];
+    switch (g.preset) {
+        case kTypeRandomize10:
+            if (preset_u(g.base[4], s) < 0.1) t = preset_map_type(preset_u(g.base[5], s), nt);
+            break;
+        case kTypeSlices:
+            t = preset_map_type((double)g.px[i] * kInvFix, nt);
+            break;
+        case kTypeOnion: {
+            const double dx = (double)g.px[i] * kInvFix - 0.5, dy = (double)g.py[i] * kInvFix - 0.5;
+            const double dz = g.dim == 3 ? (double)g.pz[i] * kInvFix : 0.0;
+            t = preset_map_type(sqrt(dx * dx + (dy * dy + dz * dz)) * 2, nt);
+            break;
+        }
+        case kTypeRotate:
+            t = (t + 1) % nt;
+            break;
+        case kTypeFlip:
+            t = nt - 1 - t;
+            break;
+        case kTypeMoreOfFirst:
+            t = preset_map_type(preset_u(g.base[4], s) * preset_u(g.base[5], s), nt);
+            break;
+        case kTypeKillStill: {
+            const double vx = g.vx[i], vy = g.vy[i], vz = g.dim == 3 ? (double)g.vz[i] : 0.0;
+            if (sqrt(vx * vx + (vy * vy + vz * vz)) < 0.01) t = nt - 1;
+            break;
+        }
+        default: {  // kTypeRandom: the uniform generator's type draw
+            const uint64_t m = splitmix_mix(g.ubase[3] + s) >> 40;
+            t = (int32_t)((m * (uint64_t)nt) >> 24);
+            break;
+        }
+    }
+    g.type[i] = t;
+}
+
 struct PosBuf {
     uint32_t *x, *y, *z;
 };

@@ -28,6 +28,9 @@ EXPORTS = (
     "plife_get_particles",
     "plife_get_count",
     "plife_generate_particles",
+    "plife_set_positions",
+    "plife_set_types",
+    "plife_generate_matrix",
     "plife_update",
     "plife_force_update_stop",
     "plife_synchronize",
@@ -44,6 +47,8 @@ EXPORTS = (
     "plife_batch_create",
     "plife_batch_destroy",
     "plife_batch_add_world",
+    "plife_batch_set_world_positions",
+    "plife_batch_set_world_types",
     "plife_batch_add_generated_world",
     "plife_batch_remove_world",
     "plife_batch_set_world_settings",
@@ -128,6 +133,9 @@ def load() -> ctypes.CDLL:
         "plife_get_particles": [vp, i32, vp, vp, vp, vp],
         "plife_get_count": [vp, ctypes.POINTER(i32)],
         "plife_generate_particles": [vp, i32, ctypes.c_uint64],
+        "plife_set_positions": [vp, i32, ctypes.c_uint64],
+        "plife_set_types": [vp, i32, ctypes.c_uint64],
+        "plife_generate_matrix": [i32, i32, ctypes.c_uint64, vp],
         "plife_update": [vp, i32],
         "plife_force_update_stop": [vp],
         "plife_synchronize": [vp],
@@ -144,6 +152,8 @@ def load() -> ctypes.CDLL:
         "plife_batch_destroy": [vp],
         "plife_batch_add_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32, vp, vp, vp,
                                   ctypes.POINTER(i32)],
+        "plife_batch_set_world_positions": [vp, i32, i32, ctypes.c_uint64],
+        "plife_batch_set_world_types": [vp, i32, i32, ctypes.c_uint64],
         "plife_batch_add_generated_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32,
                                             ctypes.c_uint64, ctypes.POINTER(i32)],
         "plife_batch_remove_world": [vp, i32],

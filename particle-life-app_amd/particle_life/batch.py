@@ -76,6 +76,20 @@ class Batch:
         self._n[wid.value] = int(n)
         return wid.value
 
+    def set_positions(self, wid: int, preset, seed: int) -> None:
+        """plife_batch_set_world_positions: a position preset for world wid."""
+        from .presets import POSITION_PRESETS, preset_index
+        check(self._lib.plife_batch_set_world_positions(
+            self._h, int(wid), preset_index(POSITION_PRESETS, preset),
+            ctypes.c_uint64(int(seed) & (2**64 - 1))))
+
+    def set_types(self, wid: int, preset, seed: int) -> None:
+        """plife_batch_set_world_types: a type preset for world wid."""
+        from .presets import TYPE_PRESETS, preset_index
+        check(self._lib.plife_batch_set_world_types(
+            self._h, int(wid), preset_index(TYPE_PRESETS, preset),
+            ctypes.c_uint64(int(seed) & (2**64 - 1))))
+
     def add_world(self, w) -> int:
         """Adds a ``worlds.World``."""
         return self.add(matrix=w.matrix, pos=w.pos, vel=w.vel, types=w.types, rmax=w.rmax,

@@ -69,6 +69,19 @@ class Engine:
         check(self._lib.plife_generate_particles(self._h, int(n), ctypes.c_uint64(int(seed) & (2**64 - 1))))
         self.n = int(n)
 
+    def set_positions(self, preset, seed: int) -> None:
+        """plife_set_positions: a position preset (name or index, the app's
+        order) for every particle, velocities 0, types kept."""
+        from .presets import POSITION_PRESETS, preset_index
+        check(self._lib.plife_set_positions(self._h, preset_index(POSITION_PRESETS, preset),
+                                            ctypes.c_uint64(int(seed) & (2**64 - 1))))
+
+    def set_types(self, preset, seed: int) -> None:
+        """plife_set_types: a type preset (name or index, the app's order)."""
+        from .presets import TYPE_PRESETS, preset_index
+        check(self._lib.plife_set_types(self._h, preset_index(TYPE_PRESETS, preset),
+                                        ctypes.c_uint64(int(seed) & (2**64 - 1))))
+
     def load_world(self, w) -> None:
         self.configure_world(w)
         self.set_state(w.pos, w.vel, w.types)

@@ -310,7 +310,15 @@ class GpuPhysics:
                                p.type.copy(), self.settings.deepCopy(), len(p), counts)
 
     def setPositions(self) -> None:
-        """Physics.setPositions (Physics.java:166-168)."""
+        """Physics.setPositions (Physics.java:166-168).  With one of the app's
+        presets as the position setter (presets.PresetPositionSetter) it runs
+        on the device for all particles at once; other setters run per
+        particle on the host."""
+        from .presets import PresetPositionSetter
+        if isinstance(self.positionSetter, PresetPositionSetter):
+            self._on_device(lambda e: e.set_positions(self.positionSetter.index,
+                                                      self.positionSetter.next_seed()))
+            return
         p = self.particles
         for i in range(len(p)):
             self._set_position(p, i)
@@ -363,7 +371,12 @@ class GpuPhysics:
             self._set_type(p, int(i))
 
     def setTypes(self) -> None:
-        """Physics.setTypes (Physics.java:509-511)."""
+        """Physics.setTypes (Physics.java:509-511); on the device with a
+        presets.PresetTypeSetter, as setPositions."""
+        from .presets import PresetTypeSetter
+        if isinstance(self.typeSetter, PresetTypeSetter):
+            self._on_device(lambda e: e.set_types(self.typeSetter.index, self.typeSetter.next_seed()))
+            return
         p = self.particles
         for i in range(len(p)):
             self._set_type(p, i)
@@ -405,6 +418,15 @@ class GpuPhysics:
                                             int(p.type[i]), self.settings.matrix.size())
 
     # -- device sync --------------------------------------------------------
+    def _on_device(self, op) -> None:
+        """Runs op(engine) on the device copy of the particles, bringing it up
+        to date first; the host view is refreshed lazily afterwards."""
+        self._push_settings()
+        if self._host_dirty:
+            self._upload()
+        op(self._engine)
+        self._device_newer = True
+
     def _push_settings(self) -> None:
         s = self.settings
         self._engine.configure(matrix=s.matrix.to_array(), rmax=s.rmax, friction=s.friction,
