@@ -276,7 +276,10 @@ def main() -> int:
 
     # the app's path: one update and a full state download (PhysicsSnapshot)
     # per frame, host-visible wall time
-    frames = 5
+    frames = 20
+    for _ in range(2):          # the first download allocates the pinned staging buffer
+        eng.step(1)
+        eng.state()
     eng.sync()
     t0 = time.perf_counter()
     for _ in range(frames):
