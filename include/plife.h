@@ -70,7 +70,10 @@ int plife_destroy(plife_physics *h);
  * row = type of the acted-on particle) + beta.  Cheap to call every frame
  * (app/Main.java:292-295 overwrites dt per frame); the matrix is re-uploaded
  * only when it changed.  ntypes in [1, 256] (the app caps it at 256,
- * app/Main.java:745). */
+ * app/Main.java:745).  The matrix may shrink below the particles' types, as
+ * Physics.setMatrixSize does before its ensureTypes (Physics.java:238-272);
+ * until particles within the new size are set, plife_update and
+ * plife_set_types fail with PLIFE_ERR_STATE. */
 int plife_set_settings(plife_physics *h, const plife_settings *s, int32_t ntypes,
                        const double *matrix);
 

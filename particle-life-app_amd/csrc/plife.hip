@@ -163,6 +163,7 @@ struct WorldHost {
     int64_t steps = 0;
     bool live = true;                  // false once removed (dropped at the next layout)
     double nb = -1.0;                  // neighbours per particle in the last measured step
+    int max_type = -1;                 // highest type of its particles (must stay < ntypes to step)
     bool sparse = false;               // stepped by the sparse-world force kernel
 };
 
@@ -956,6 +957,13 @@ void classify_worlds(plife_physics *h)
 int prepare_step(plife_physics *h)
 {
     int rc;
+    // a matrix made smaller than the particles' types (plife_set_settings) needs
+    // particles within it before the next step: the force law indexes the matrix by type
+    for (const WorldHost &w : h->worlds)
+        if (w.live && w.n > 0 && w.max_type >= w.ntypes)
+            return fail(PLIFE_ERR_STATE,
+                        "world %d: particle types reach %d but the matrix has %d types; set particles "
+                        "with types below the matrix size before stepping", w.id, w.max_type, w.ntypes);
     if ((rc = sync_layout(h))) return rc;
     refresh_occupancy(h);
     classify_worlds(h);
@@ -1369,6 +1377,9 @@ int apply_preset(plife_physics *h, WorldHost &w, bool positions, int preset, uin
     if (preset < 0 || preset >= (positions ? (int)kPosPresets : (int)kTypePresets))
         return fail(PLIFE_ERR_ARGUMENT, "%s preset %d out of range", positions ? "position" : "type", preset);
     if (w.n == 0) return PLIFE_OK;
+    if (!positions && w.max_type >= w.ntypes)
+        return fail(PLIFE_ERR_STATE, "particle types reach %d but the matrix has %d types; set the particles first",
+                    w.max_type, w.ntypes);
     PresetArgs g{};
     for (int s = 0; s < 6; ++s) g.base[s] = splitmix_mix(seed + (uint64_t)(kPresetStream + s) * kStream);
     for (int s = 0; s < 4; ++s) g.ubase[s] = splitmix_mix(seed + (uint64_t)(s + 1) * kStream);
@@ -1391,6 +1402,7 @@ int apply_preset(plife_physics *h, WorldHost &w, bool positions, int preset, uin
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(h->stream));
     w.nb = -1.0;             // the world's clustering is no longer known
+    if (!positions) w.max_type = w.ntypes - 1;
     h->stepped = false;      // cell by-products describe the old state
     return PLIFE_OK;
 }
@@ -1643,6 +1655,7 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
     if ((rc = upload_particles(h, 0, n, pos, vel, types, w.st.wrap != 0, w.st.dim == 3))) return rc;
     w.n = n;
     w.off = 0;
+    w.max_type = n > 0 ? *std::max_element(types, types + n) : -1;
     h->used = n;
     h->layout_dirty = true;
     h->stepped = false;
@@ -1662,6 +1675,7 @@ int plife_generate_particles(plife_physics *h, int32_t n, uint64_t seed)
     if ((rc = generate_particles(h, 0, n, seed, w.ntypes, w.st.dim))) return rc;
     w.n = n;
     w.off = 0;
+    w.max_type = n > 0 ? w.ntypes - 1 : -1;
     h->used = n;
     h->layout_dirty = true;
     h->stepped = false;
@@ -1918,6 +1932,7 @@ int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntype
     if ((rc = ensure_capacity(b, b->used + n, true))) return rc;
     if ((rc = upload_particles(b, b->used, n, pos, vel, types, s->wrap != 0, s->dim == 3))) return rc;
     WorldHost w;
+    w.max_type = n > 0 ? *std::max_element(types, types + n) : -1;
     w.id = b->next_id++;
     w.st = *s;
     w.ntypes = ntypes;
@@ -1974,6 +1989,7 @@ int plife_batch_add_generated_world(plife_batch *b, const plife_settings *s, int
     if ((rc = ensure_capacity(b, b->used + n, true))) return rc;
     if ((rc = generate_particles(b, b->used, n, seed, ntypes, s->dim))) return rc;
     WorldHost w;
+    w.max_type = n > 0 ? ntypes - 1 : -1;
     w.id = b->next_id++;
     w.st = *s;
     w.ntypes = ntypes;

@@ -89,3 +89,33 @@ def test_save_load_round_trip_steps_identically():
     np.testing.assert_array_equal(pa.velocity, pb.velocity)
     a.kill()
     b.kill()
+
+
+def test_matrix_shrunk_below_types_blocks_steps_until_fixed():
+    """plife_set_settings may shrink the matrix below the particles' types (as
+    Physics.setMatrixSize does before ensureTypes); stepping is refused with a
+    state error until particles within the new size are set."""
+    import numpy as np
+    from particle_life import Engine, worlds
+    from particle_life._native import PlifeError
+
+    print("\n[physics] matrix shrunk below types", flush=True)
+    w = worlds.uniform_world(401, 2_000, 6, rmax=0.03, dt=0.01)
+    e = Engine(0)
+    try:
+        e.load_world(w)
+        e.step(1)
+        small = w.matrix[:3, :3].copy()
+        e.configure(matrix=small, rmax=w.rmax, friction=w.friction, dt=w.dt, beta=w.beta,
+                    force=w.force, wrap=w.wrap, dim=w.dim)
+        with pytest.raises(PlifeError):
+            e.step(1)
+        with pytest.raises(PlifeError):
+            e.set_types("rotate", 1)
+        st = e.state()
+        e.set_state(st["pos"], st["vel"], np.minimum(st["types"], 2))
+        e.step(2)
+        e.set_types("flip", 2)
+        assert e.state()["types"].max() <= 2
+    finally:
+        e.close()
