@@ -202,6 +202,13 @@ struct plife_physics {
     int blocks_bs = 0, nb_total = 0;
     int seg[4] = {};                   // blocks of (periodic, clamped) x (staged, sparse) worlds
     uint64_t blocks_version = ~0ull, blocks_classes = ~0ull;
+    uint64_t blocks_serial = 0;        // bumped whenever the table's content or order changes
+    std::vector<uint64_t> block_key;   // per entry: world id, sparse kernel, first home in the world
+    std::vector<uint32_t> block_est;   // per entry: expected pairs (before a step measures them)
+    uint32_t *d_block_work = nullptr;  // pairs of each table entry in the last step, + tag
+    size_t block_work_cap = 0;
+    uint32_t *work_host = nullptr;     // pinned copy of d_block_work
+    size_t work_host_cap = 0;
     uint64_t classes = 0;              // bumped whenever a world changes force kernel
 
     // particle state: positions ping-pong P[2], the rest A[2]
@@ -595,7 +602,17 @@ int ensure_blocks(plife_physics *h, int bs)
 {
     if (h->blocks_bs == bs && h->blocks_version == h->version && h->blocks_classes == h->classes)
         return PLIFE_OK;
+    // the last step's work of the blocks that survive, by block identity
+    std::unordered_map<uint64_t, uint32_t> prev;
+    const int nold = (int)h->blocks.size();
+    if (nold > 0 && h->work_host && h->work_host_cap > (size_t)nold &&
+        (int)h->block_key.size() == nold &&
+        __atomic_load_n(h->work_host + nold, __ATOMIC_RELAXED) == (uint32_t)(h->blocks_serial + 1))
+        for (int i = 0; i < nold; ++i)
+            prev[h->block_key[i]] = __atomic_load_n(h->work_host + i, __ATOMIC_RELAXED);
     h->blocks.clear();
+    h->block_key.clear();
+    h->block_est.clear();
     for (int s = 0; s < 4; ++s) {
         const bool wrap = s < 2, sparse = (s & 1) != 0;
         const size_t before = h->blocks.size();
@@ -603,14 +620,39 @@ int ensure_blocks(plife_physics *h, int bs)
             const WorldHost &wh = h->worlds[w];
             if ((wh.st.wrap != 0) != wrap || wh.sparse != sparse) continue;
             const int step = sparse ? kSparseBlock : bs;
-            for (int g = 0; g < wh.n; g += step)
+            // expected pairs per home where no measurement carries over
+            const double r = wh.st.rmax;
+            const double ball = wh.st.dim == 3 ? 4.0 / 3.0 * M_PI * r * r * r : M_PI * r * r;
+            const double nb = wh.nb >= 0.0 ? wh.nb : std::max(wh.n - 1, 0) * ball;
+            for (int g = 0; g < wh.n; g += step) {
+                const uint64_t key = ((uint64_t)(uint32_t)wh.id << 33) | ((uint64_t)sparse << 32) | (uint32_t)g;
+                const auto it = prev.find(key);
+                const double homes = std::min(step, wh.n - g);
                 h->blocks.push_back(make_uint2((uint32_t)w, (uint32_t)(wh.off + g)));
+                h->block_key.push_back(key);
+                h->block_est.push_back(it != prev.end() ? it->second
+                                                        : (uint32_t)std::min(nb * homes, 4.0e9));
+            }
         }
         h->seg[s] = (int)(h->blocks.size() - before);
     }
     h->nb_total = (int)h->blocks.size();
     int rc;
     if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size(), &h->retired))) return rc;
+    if ((rc = ensure_dev(&h->d_block_work, &h->block_work_cap, h->blocks.size() + 1, &h->retired)))
+        return rc;
+    if (h->blocks.size() + 1 > h->work_host_cap) {
+        // the old buffer may still be written by queued steps: keep it until destroy
+        const size_t cap = grown(h->work_host_cap, h->blocks.size() + 1);
+        uint32_t *p = nullptr;
+        const hipError_t e = hipHostMalloc((void **)&p, cap * 4, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
+        std::memset(p, 0, cap * 4);
+        if (h->work_host) h->retired_host.push_back(h->work_host);
+        h->work_host = p;
+        h->work_host_cap = cap;
+    }
+    ++h->blocks_serial;
     if (!h->blocks.empty()) {
         const size_t bytes = h->blocks.size() * sizeof(uint2);
         if ((rc = begin_uploads(h, bytes))) return rc;
@@ -909,6 +951,73 @@ void refresh_occupancy(plife_physics *h)
     else if (h->clustered && (ratio < 3.0 || h->occupancy < 48.0)) h->clustered = false;
 }
 
+// Table position of the block dispatched as blockIdx b of a launch of nb
+// blocks (the kernels' xcd_remap).
+inline int remap_pos(int b, int nb)
+{
+    const int q = nb >> 3, rem = nb & 7, xcd = b & 7, k = b >> 3;
+    return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+}
+
+// Orders each segment of the block table heaviest first when the work of its
+// blocks (their pairs in the last measured step) is skewed and a heavy block
+// is dispatched late: workgroups start in dispatch order, so a heavy block
+// near the end keeps a few CUs busy after the rest have drained.  Blocks are
+// dealt in decreasing work round-robin to the 8 XCDs (blockIdx i -> table
+// position remap_pos(i)).  Results do not depend on the order.
+int balance_blocks(plife_physics *h)
+{
+    const int nb = h->nb_total;
+    if (nb < 64 || (int)h->block_est.size() != nb) return PLIFE_OK;
+    // the last step's measurement of this table, else the estimates it was built with
+    std::vector<uint32_t> work(h->block_est);
+    if (h->work_host && h->work_host_cap > (size_t)nb &&
+        __atomic_load_n(h->work_host + nb, __ATOMIC_RELAXED) == (uint32_t)(h->blocks_serial + 1))
+        for (int i = 0; i < nb; ++i) work[i] = __atomic_load_n(h->work_host + i, __ATOMIC_RELAXED);
+    bool changed = false;
+    int base = 0;
+    for (int s = 0; s < 4; ++s) {
+        const int n = h->seg[s];
+        const int b0 = base;
+        base += n;
+        if (n < 64) continue;
+        double mean = 0.0;
+        uint32_t late = 0;              // heaviest block dispatched in the second half
+        for (int b = 0; b < n; ++b) {
+            const uint32_t wk = work[b0 + remap_pos(b, n)];
+            mean += wk;
+            if (b >= n / 2) late = std::max(late, wk);
+        }
+        mean /= n;
+        if (!(late > 2.0 * mean) || !(mean > 0.0)) continue;
+        std::vector<int> order(n);
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return work[b0 + x] > work[b0 + y]; });
+        std::vector<uint2> blk(n);
+        std::vector<uint64_t> key(n);
+        std::vector<uint32_t> est(n);
+        for (int i = 0; i < n; ++i) {
+            const int to = remap_pos(i, n), from = b0 + order[i];
+            blk[to] = h->blocks[from];
+            key[to] = h->block_key[from];
+            est[to] = work[from];
+        }
+        std::copy(blk.begin(), blk.end(), h->blocks.begin() + b0);
+        std::copy(key.begin(), key.end(), h->block_key.begin() + b0);
+        std::copy(est.begin(), est.end(), h->block_est.begin() + b0);
+        changed = true;
+    }
+    if (!changed) return PLIFE_OK;
+    int rc;
+    const size_t bytes = h->blocks.size() * sizeof(uint2);
+    if ((rc = begin_uploads(h, bytes))) return rc;
+    if ((rc = upload(h, h->d_blocks, h->blocks.data(), bytes))) return rc;
+    if ((rc = end_uploads(h))) return rc;
+    ++h->blocks_serial;
+    return PLIFE_OK;
+}
+
 // Which force kernel steps each world.  A world goes to the sparse-world
 // kernel when a staged workgroup of its homes would span more than two grid
 // rows or more cells than the staged cell table holds; the span follows from
@@ -977,7 +1086,8 @@ int prepare_step(plife_physics *h)
     if ((rc = ensure_dev(&h->bsum, &h->bsum_cap, nb, &h->retired))) return rc;
     if ((rc = ensure_dev(&h->bsq, &h->bsq_cap, (size_t)ceil_div(h->ncells, kScanTile), &h->retired)))
         return rc;
-    return ensure_blocks(h, h->plan.bs);
+    if ((rc = ensure_blocks(h, h->plan.bs))) return rc;
+    return balance_blocks(h);
 }
 
 // One step of every world.  prepare_step must have run since the last change.
@@ -1069,6 +1179,9 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.max_cells = fp.max_cells;
     a.pair_total = h->d_pairs;
     a.pair_world = h->d_pairs + 2;
+    a.block_work = h->d_block_work;
+    a.work_tag = (uint32_t)(h->blocks_serial + 1);
+    a.nb_total = h->nb_total;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
     h->last_lds = fp.lds;
@@ -1079,6 +1192,8 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     // the step's counters (pairs, clustering measure, per-world pairs, layout
     // tag) travel to pinned host memory for the launch planner
     HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs, (3 + h->worlds.size()) * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->work_host, h->d_block_work, ((size_t)h->nb_total + 1) * 4,
                            hipMemcpyDeviceToHost, h->stream));
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
@@ -1095,7 +1210,8 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
         plife_tuning tune;
         ForcePlan plan;
         int seg[4];
-        const void *ptr[36];
+        uint64_t blocks_serial;
+        const void *ptr[40];
     } k;
     std::memset(&k, 0, sizeof k);
     k.n = h->n;
@@ -1108,12 +1224,13 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     k.tune = h->tune;
     k.plan = h->plan;
     std::memcpy(k.seg, h->seg, sizeof k.seg);
+    k.blocks_serial = h->blocks_serial;
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
                        h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
                        h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash,
                        h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks,
-                       h->stat_host};
+                       h->stat_host, h->d_block_work, h->work_host};
     std::memcpy(k.ptr, p, sizeof p);
     std::vector<unsigned char> v(sizeof k);
     std::memcpy(v.data(), &k, sizeof k);
@@ -1524,9 +1641,11 @@ void destroy_handle(plife_physics *h)
     dev_free(h->world_of);
     dev_free(h->d_src_off);
     dev_free(h->d_blocks);
+    dev_free(h->d_block_work);
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
     if (h->stat_host) (void)hipHostFree(h->stat_host);
+    if (h->work_host) (void)hipHostFree(h->work_host);
     for (void *p : h->retired_host) (void)hipHostFree(p);
     h->retired_host.clear();
     if (h->pin) (void)hipHostFree(h->pin);

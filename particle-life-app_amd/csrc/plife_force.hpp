@@ -49,6 +49,9 @@ struct ForceArgs {
     int cap, lcap, max_cells;
     unsigned long long *pair_total;  // interacting pairs of the step, all worlds
     unsigned long long *pair_world;  // per world
+    uint32_t *block_work;            // per block-table entry: its pairs (saturated); [nb_total] = tag
+    uint32_t work_tag;               // identifies the block table the counts belong to
+    int nb_total;
     int32_t *dbg_count;
     unsigned long long *dbg_hash;
 };
@@ -103,6 +106,16 @@ __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, doub
         acc.cnt += 1;
         if constexpr (DEBUG) acc.hash += id_hash(qid);
     }
+}
+
+// The block's pairs, for the host to order the next updates' blocks
+// heaviest first; entry nb_total tags the table they belong to.
+__device__ __forceinline__ void record_block_work(const ForceArgs &a, int blk,
+                                                  unsigned long long pairs)
+{
+    const int e = a.block_base + blk;
+    a.block_work[e] = (uint32_t)min(pairs, 0xFFFFFFFFull);
+    if (e == 0) a.block_work[a.nb_total] = a.work_tag;
 }
 
 __device__ __forceinline__ int wave_max(int v)
@@ -695,6 +708,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             atomicAdd(a.pair_total, sum);
             atomicAdd(a.pair_world + bw.x, sum);
         }
+        record_block_work(a, blk, sum);
     }
 }
 
@@ -749,6 +763,7 @@ __global__ __launch_bounds__(kSparseBlock) void k_force_sparse(ForceArgs a)
             atomicAdd(a.pair_total, sum);
             atomicAdd(a.pair_world + bw.x, sum);
         }
+        record_block_work(a, blk, sum);
     }
 }
 
