@@ -144,3 +144,42 @@ def test_batch_live_settings_and_tuning():
         b.close()
     for a, c in zip(*res):
         _assert_same(a, c, "tuning")
+
+
+def test_work_balanced_launches_equal_worlds_alone():
+    """A clustered world among uniform ones makes the block table's work
+    skewed, so later updates dispatch its blocks heaviest first (and across
+    layout changes, with the work carried over); every world still equals the
+    same world stepped alone in one update."""
+    from particle_life import Batch, Engine, worlds
+
+    print("\n[batch] case=work_balanced", flush=True)
+    ws = [worlds.gaussian_blob_world(520, 40_000, 6, rmax=0.02, dt=0.01, beta=0.3, nblobs=4,
+                                     sigma=0.01, dim=2)]
+    ws += [worlds.uniform_world(521 + k, 20_000, 5, rmax=0.015, dt=0.01) for k in range(4)]
+    b = Batch(0)
+    try:
+        ids = [b.add_world(w) for w in ws]
+        for _ in range(3):
+            b.step(1)
+            b.sync()
+        extra = b.add_world(worlds.uniform_world(530, 5_000, 4, rmax=0.02, dt=0.01))
+        for _ in range(3):
+            b.step(1)
+            b.sync()
+        b.remove(extra)
+        b.step(1)
+        b.sync()
+        got = [b.state(i) for i in ids]
+    finally:
+        b.close()
+    e = Engine(0)
+    try:
+        for w, g in zip(ws, got):
+            e.load_world(w)
+            e.step(7)
+            s = e.state()
+            for key in ("pos", "vel", "types", "ids"):
+                np.testing.assert_array_equal(g[key], s[key], err_msg=key)
+    finally:
+        e.close()
