@@ -87,6 +87,15 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
 /* Downloads the state in the current device array order (cell-sorted after
  * a step, like Physics.particles after makeContainers' swap).  Any output
  * pointer may be NULL.  Synchronises the handle's stream. */
+/* In-place edits between steps -- the app's cursor tools change a few
+ * particles (app/Main.java:543-580) -- without re-uploading the rest: the
+ * particle at slots[i] of the current order (the order plife_get_particles
+ * returns) gets pos[3i..3i+2], vel[3i..3i+2] and types[i], wrapped or
+ * clamped as plife_set_particles does; ids are kept.  Slots must be distinct
+ * and in [0, n), types in [0, ntypes). */
+int plife_set_particle_slots(plife_physics *h, int32_t k, const int32_t *slots, const double *pos,
+                             const double *vel, const int32_t *types);
+
 int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
                         int32_t *ids);
 
@@ -216,7 +225,11 @@ int plife_batch_destroy(plife_batch *b);
 int plife_batch_add_world(plife_batch *b, const plife_settings *s, int32_t ntypes,
                           const double *matrix, int32_t n, const double *pos, const double *vel,
                           const int32_t *types, int32_t *world_id);
-/* plife_set_positions / plife_set_types for one world of the batch. */
+/* plife_set_particle_slots / plife_set_positions / plife_set_types for one
+ * world of the batch (slots of the world's own order, as plife_batch_get_world
+ * returns it). */
+int plife_batch_set_world_slots(plife_batch *b, int32_t world_id, int32_t k, const int32_t *slots,
+                                const double *pos, const double *vel, const int32_t *types);
 int plife_batch_set_world_positions(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed);
 int plife_batch_set_world_types(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed);
 

@@ -241,6 +241,8 @@ struct plife_physics {
     int up_cur = 0;
     size_t up_used = 0;
     std::vector<void *> retired;       // replaced small device arrays, freed at destroy
+    SlotEdit *d_edits = nullptr;       // in-place particle edits of the last plife_set_particle_slots
+    size_t edits_cap = 0;
     double occupancy = 0.0;            // particle-weighted cell occupancy seen by the planner
     bool clustered = false;            // the planner's regime (with hysteresis)
 
@@ -1524,6 +1526,55 @@ int apply_preset(plife_physics *h, WorldHost &w, bool positions, int preset, uin
     return PLIFE_OK;
 }
 
+// Overwrites k particles of world w at slots of its current order (the
+// order downloads return), keeping their ids; the layout must be synced.
+int set_slots(plife_physics *h, WorldHost &w, int32_t k, const int32_t *slots, const double *pos,
+              const double *vel, const int32_t *types)
+{
+    if (k < 0) return fail(PLIFE_ERR_ARGUMENT, "k must be >= 0, got %d", k);
+    if (k == 0) return PLIFE_OK;
+    if (!slots || !pos || !vel || !types) return fail(PLIFE_ERR_ARGUMENT, "null array");
+    std::vector<uint8_t> seen((size_t)std::max(w.n, 1), 0);
+    std::vector<SlotEdit> e((size_t)k);
+    const bool wrap = w.st.wrap != 0, d3 = w.st.dim == 3;
+    int tmax = -1;
+    for (int i = 0; i < k; ++i) {
+        const int s = slots[i];
+        if (s < 0 || s >= w.n) return fail(PLIFE_ERR_ARGUMENT, "slot %d outside [0, %d)", s, w.n);
+        if (seen[s]) return fail(PLIFE_ERR_ARGUMENT, "slot %d given twice", s);
+        seen[s] = 1;
+        if (types[i] < 0 || types[i] >= w.ntypes)
+            return fail(PLIFE_ERR_ARGUMENT, "edit %d has type %d outside [0, %d)", i, types[i], w.ntypes);
+        e[i] = SlotEdit{s, fixed_of(pos[3 * i], wrap), fixed_of(pos[3 * i + 1], wrap),
+                        d3 ? fixed_of(pos[3 * i + 2], wrap) : 0u, (float)vel[3 * i], (float)vel[3 * i + 1],
+                        d3 ? (float)vel[3 * i + 2] : 0.0f, types[i]};
+        tmax = std::max(tmax, types[i]);
+    }
+    int rc;
+    if ((rc = ensure_dev(&h->d_edits, &h->edits_cap, (size_t)k, &h->retired))) return rc;
+    const size_t bytes = (size_t)k * sizeof(SlotEdit);
+    if ((rc = begin_uploads(h, bytes))) return rc;
+    if ((rc = upload(h, h->d_edits, e.data(), bytes))) return rc;
+    if ((rc = end_uploads(h))) return rc;
+    const int p = h->pc, a = h->ac;
+    k_set_slots<<<ceil_div(k, kBlock), kBlock, 0, h->stream>>>(h->d_edits, k, w.off, h->px[p], h->py[p],
+                                                               h->pz[p], h->vx[a], h->vy[a], h->vz[a],
+                                                               h->type[a]);
+    LAUNCH_CHECK();
+    h->stepped = false;
+    if (w.max_type < w.ntypes) {
+        w.max_type = std::max(w.max_type, tmax);
+        return PLIFE_OK;
+    }
+    // a matrix shrunk below the types: the edits may be what brings them
+    // back within it, so take the true maximum
+    std::vector<int32_t> all((size_t)w.n);
+    HIP_TRY(hipMemcpyAsync(all.data(), h->type[a] + w.off, (size_t)w.n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    w.max_type = *std::max_element(all.begin(), all.end());
+    return PLIFE_OK;
+}
+
 // Runs f(lo, hi) over [0, n) on up to 8 host threads (one for small n).
 template <typename F>
 void parallel_chunks(int n, F f)
@@ -1642,6 +1693,7 @@ void destroy_handle(plife_physics *h)
     dev_free(h->d_src_off);
     dev_free(h->d_blocks);
     dev_free(h->d_block_work);
+    dev_free(h->d_edits);
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
     if (h->stat_host) (void)hipHostFree(h->stat_host);
@@ -1811,6 +1863,16 @@ int plife_set_positions(plife_physics *h, int32_t preset, uint64_t seed)
     if ((rc = apply_preset(h, h->worlds[0], true, preset, seed))) return rc;
     reset_planner(h);
     return PLIFE_OK;
+}
+
+int plife_set_particle_slots(plife_physics *h, int32_t k, const int32_t *slots, const double *pos,
+                             const double *vel, const int32_t *types)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = set_device(h))) return rc;
+    if ((rc = sync_layout(h))) return rc;
+    return set_slots(h, h->worlds[0], k, slots, pos, vel, types);
 }
 
 int plife_set_types(plife_physics *h, int32_t preset, uint64_t seed)
@@ -2075,6 +2137,17 @@ int plife_batch_set_world_positions(plife_batch *b, int32_t world_id, int32_t pr
     if (rc) return rc;
     if ((rc = sync_layout(b))) return rc;
     return apply_preset(b, *find_world(b, world_id), true, preset, seed);
+}
+
+int plife_batch_set_world_slots(plife_batch *b, int32_t world_id, int32_t k, const int32_t *slots,
+                                const double *pos, const double *vel, const int32_t *types)
+{
+    if (!b) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!find_world(b, world_id)) return fail(PLIFE_ERR_ARGUMENT, "no world %d in the batch", world_id);
+    int rc = set_device(b);
+    if (rc) return rc;
+    if ((rc = sync_layout(b))) return rc;
+    return set_slots(b, *find_world(b, world_id), k, slots, pos, vel, types);
 }
 
 int plife_batch_set_world_types(plife_batch *b, int32_t world_id, int32_t preset, uint64_t seed)

@@ -656,6 +656,32 @@ __global__ __launch_bounds__(kBlock) void k_set_types(PresetArgs g)
     g.type[i] = t;
 }
 
+// One in-place edit of a particle between steps (positions already wrapped or
+// clamped and rounded to the 2^-32 grid); ids are kept.
+struct SlotEdit {
+    int32_t slot;
+    uint32_t x, y, z;
+    float vx, vy, vz;
+    int32_t type;
+};
+
+__global__ __launch_bounds__(kBlock) void k_set_slots(const SlotEdit *__restrict__ e, int k, int off,
+                                                      uint32_t *px, uint32_t *py, uint32_t *pz,
+                                                      float *vx, float *vy, float *vz, int32_t *type)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= k) return;
+    const SlotEdit r = e[i];
+    const int s = off + r.slot;
+    px[s] = r.x;
+    py[s] = r.y;
+    pz[s] = r.z;
+    vx[s] = r.vx;
+    vy[s] = r.vy;
+    vz[s] = r.vz;
+    type[s] = r.type;
+}
+
 struct PosBuf {
     uint32_t *x, *y, *z;
 };

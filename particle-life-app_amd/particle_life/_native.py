@@ -25,6 +25,7 @@ EXPORTS = (
     "plife_destroy",
     "plife_set_settings",
     "plife_set_particles",
+    "plife_set_particle_slots",
     "plife_get_particles",
     "plife_get_count",
     "plife_generate_particles",
@@ -47,6 +48,7 @@ EXPORTS = (
     "plife_batch_create",
     "plife_batch_destroy",
     "plife_batch_add_world",
+    "plife_batch_set_world_slots",
     "plife_batch_set_world_positions",
     "plife_batch_set_world_types",
     "plife_batch_add_generated_world",
@@ -130,6 +132,7 @@ def load() -> ctypes.CDLL:
         "plife_destroy": [vp],
         "plife_set_settings": [vp, ctypes.POINTER(PlifeSettings), i32, vp],
         "plife_set_particles": [vp, i32, vp, vp, vp],
+        "plife_set_particle_slots": [vp, i32, vp, vp, vp, vp],
         "plife_get_particles": [vp, i32, vp, vp, vp, vp],
         "plife_get_count": [vp, ctypes.POINTER(i32)],
         "plife_generate_particles": [vp, i32, ctypes.c_uint64],
@@ -152,6 +155,7 @@ def load() -> ctypes.CDLL:
         "plife_batch_destroy": [vp],
         "plife_batch_add_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32, vp, vp, vp,
                                   ctypes.POINTER(i32)],
+        "plife_batch_set_world_slots": [vp, i32, i32, vp, vp, vp, vp],
         "plife_batch_set_world_positions": [vp, i32, i32, ctypes.c_uint64],
         "plife_batch_set_world_types": [vp, i32, i32, ctypes.c_uint64],
         "plife_batch_add_generated_world": [vp, ctypes.POINTER(PlifeSettings), i32, vp, i32,

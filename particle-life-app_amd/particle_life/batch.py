@@ -76,6 +76,20 @@ class Batch:
         self._n[wid.value] = int(n)
         return wid.value
 
+    def set_slots(self, wid: int, slots, pos, vel, types) -> None:
+        """plife_batch_set_world_slots: overwrite particles of world wid at
+        `slots` of its current order, keeping their ids."""
+        slots = np.ascontiguousarray(slots, dtype=np.int32).reshape(-1)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        k = int(slots.shape[0])
+        if pos.shape[0] != k or vel.shape[0] != k or types.shape[0] != k:
+            raise ValueError("slots, pos, vel and types must have the same length")
+        check(self._lib.plife_batch_set_world_slots(self._h, int(wid), k, slots.ctypes.data,
+                                                    pos.ctypes.data, vel.ctypes.data,
+                                                    types.ctypes.data))
+
     def set_positions(self, wid: int, preset, seed: int) -> None:
         """plife_batch_set_world_positions: a position preset for world wid."""
         from .presets import POSITION_PRESETS, preset_index

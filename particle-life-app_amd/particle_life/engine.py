@@ -69,6 +69,19 @@ class Engine:
         check(self._lib.plife_generate_particles(self._h, int(n), ctypes.c_uint64(int(seed) & (2**64 - 1))))
         self.n = int(n)
 
+    def set_slots(self, slots, pos, vel, types) -> None:
+        """plife_set_particle_slots: overwrite the particles at `slots` of the
+        current order (the order state() returns), keeping their ids."""
+        slots = np.ascontiguousarray(slots, dtype=np.int32).reshape(-1)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        k = int(slots.shape[0])
+        if pos.shape[0] != k or vel.shape[0] != k or types.shape[0] != k:
+            raise ValueError("slots, pos, vel and types must have the same length")
+        check(self._lib.plife_set_particle_slots(self._h, k, slots.ctypes.data, pos.ctypes.data,
+                                                 vel.ctypes.data, types.ctypes.data))
+
     def set_positions(self, preset, seed: int) -> None:
         """plife_set_positions: a position preset (name or index, the app's
         order) for every particle, velocities 0, types kept."""

@@ -309,6 +309,28 @@ class GpuPhysics:
         return PhysicsSnapshot(p.position.reshape(-1).copy(), p.velocity.reshape(-1).copy(),
                                p.type.copy(), self.settings.deepCopy(), len(p), counts)
 
+    def editParticles(self, indices, position=None, velocity=None, type_=None) -> None:
+        """In-place edits of some particles between steps (the app's cursor
+        tools, app/Main.java:543-580): the given rows of positions, velocities
+        and/or types change in the host view and only those rows go to the
+        device (plife_set_particle_slots).  Positions are wrapped or clamped
+        (ensurePosition) on the way."""
+        self._download()
+        p = self._particles
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if position is not None:
+            p.position[idx] = np.asarray(position, dtype=np.float64).reshape(-1, 3)
+            for i in idx:
+                self.ensurePosition(p.position[i])
+        if velocity is not None:
+            p.velocity[idx] = np.asarray(velocity, dtype=np.float64).reshape(-1, 3)
+        if type_ is not None:
+            p.type[idx] = np.asarray(type_, dtype=np.int32).reshape(-1)
+        if self._host_dirty:
+            return                   # the whole view goes up at the next update anyway
+        self._push_settings()
+        self._engine.set_slots(idx, p.position[idx], p.velocity[idx], p.type[idx])
+
     def setPositions(self) -> None:
         """Physics.setPositions (Physics.java:166-168).  With one of the app's
         presets as the position setter (presets.PresetPositionSetter) it runs

@@ -119,3 +119,93 @@ def test_matrix_shrunk_below_types_blocks_steps_until_fixed():
         assert e.state()["types"].max() <= 2
     finally:
         e.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_slot_edits_step_like_a_full_upload(dim):
+    """plife_set_particle_slots changes only the edited rows (ids kept), and the
+    edited state steps bit-identically to the same state uploaded whole."""
+    import numpy as np
+    from particle_life import Engine, worlds
+    from particle_life._native import PlifeError
+
+    print(f"\n[physics] slot edits dim={dim}", flush=True)
+    w = worlds.uniform_world(410 + dim, 6_000, 5, rmax=0.03 if dim == 2 else 0.06, dt=0.01, dim=dim)
+    rng = np.random.default_rng(7)
+    a, b = Engine(0), Engine(0)
+    try:
+        a.load_world(w)
+        a.step(3)
+        s0 = a.state()
+        slots = rng.choice(w.n, 64, replace=False).astype(np.int32)
+        pos = rng.random((64, 3)) * 1.2 - 0.1          # some outside the box: wrapped
+        if dim == 2:
+            pos[:, 2] = 0.0
+        vel = (rng.random((64, 3)) - 0.5) * 0.01
+        if dim == 2:
+            vel[:, 2] = 0.0
+        typ = rng.integers(0, 5, 64).astype(np.int32)
+        a.set_slots(slots, pos, vel, typ)
+        s1 = a.state()
+        keep = np.ones(w.n, bool)
+        keep[slots] = False
+        for k in ("pos", "vel", "types", "ids"):
+            np.testing.assert_array_equal(s1[k][keep], s0[k][keep])
+        np.testing.assert_array_equal(s1["ids"], s0["ids"])
+        np.testing.assert_array_equal(s1["types"][slots], typ)
+        np.testing.assert_array_equal(s1["vel"][slots], vel.astype(np.float32).astype(np.float64))
+        want = np.round((pos - np.floor(pos)) * 2.0**32) % 2.0**32 * 2.0**-32
+        np.testing.assert_array_equal(s1["pos"][slots], want)
+        b.configure_world(w)
+        b.set_state(s1["pos"], s1["vel"], s1["types"])
+        a.step(3)
+        b.step(3)
+        ra, rb = a.state(), b.state()
+        for k in ("pos", "vel", "types"):
+            np.testing.assert_array_equal(ra[k], rb[k])
+        np.testing.assert_array_equal(s1["ids"][rb["ids"]], ra["ids"])
+        for bad in ([0, 0], [w.n], [-1]):
+            with pytest.raises(PlifeError):
+                a.set_slots(bad, np.zeros((len(bad), 3)), np.zeros((len(bad), 3)),
+                            np.zeros(len(bad), np.int32))
+        with pytest.raises(PlifeError):
+            a.set_slots([1], np.zeros((1, 3)), np.zeros((1, 3)), [5])
+    finally:
+        a.close()
+        b.close()
+
+
+def test_batch_slot_edits_and_gpu_physics_edits():
+    import numpy as np
+    from particle_life import Batch, GpuPhysics, ParticleLifeAccelerator, worlds
+
+    print("\n[physics] batch slot edits, editParticles", flush=True)
+    ws = [worlds.uniform_world(420 + k, 3_000, 4, rmax=0.03, dt=0.01) for k in range(3)]
+    bt = Batch(0)
+    try:
+        ids = [bt.add_world(w) for w in ws]
+        bt.step(2)
+        bt.sync()
+        before = [bt.state(i) for i in ids]
+        bt.set_slots(ids[1], [5, 17], [[0.25, 0.5, 0], [0.75, 0.125, 0]], np.zeros((2, 3)), [3, 0])
+        after = [bt.state(i) for i in ids]
+        for k in (0, 2):
+            for key in ("pos", "vel", "types", "ids"):
+                np.testing.assert_array_equal(after[k][key], before[k][key])
+        np.testing.assert_array_equal(after[1]["pos"][[5, 17]], [[0.25, 0.5, 0], [0.75, 0.125, 0]])
+        np.testing.assert_array_equal(after[1]["types"][[5, 17]], [3, 0])
+    finally:
+        bt.close()
+    phys = GpuPhysics(ParticleLifeAccelerator(0.3), particleCount=2_000)
+    try:
+        phys.update(2)
+        phys.snapshot()
+        phys.editParticles([0, 1], position=[[0.5, 0.5, 0], [1.25, -0.5, 0]], type_=[1, 2])
+        snap = phys.snapshot()
+        np.testing.assert_array_equal(snap.positions[:6], [0.5, 0.5, 0, 0.25, 0.5, 0])
+        st = phys._engine.state()
+        np.testing.assert_array_equal(st["pos"][:2], [[0.5, 0.5, 0], [0.25, 0.5, 0]])
+        np.testing.assert_array_equal(st["types"][:2], [1, 2])
+        phys.update(2)
+    finally:
+        phys.kill()
