@@ -697,6 +697,11 @@ int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
     if (a.n <= 0) std::swap(a, b);
     if (a.n <= 0) return PLIFE_OK;
     const int nb0 = ceil_div(a.n, kScanTile), nb1 = b.n > 0 ? ceil_div(b.n, kScanTile) : 0;
+    if (a.n <= kSmallScan && b.n <= kSmallScan) {
+        k_scan_small<<<nb1 ? 2 : 1, kSmallBlock, 0, h->stream>>>(a, b);
+        LAUNCH_CHECK();
+        return PLIFE_OK;
+    }
     if ((size_t)(nb0 + nb1) > h->bsum_cap || !h->bsum)
         return fail(PLIFE_ERR_STATE, "scan scratch was not prepared");
     if ((a.sq && (!h->bsq || (size_t)nb0 > h->bsq_cap)) || b.sq)

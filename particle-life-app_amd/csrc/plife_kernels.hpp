@@ -44,7 +44,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
-// Exclusive scan over a 256-thread block.  s_w: 4 words of LDS.
+// Exclusive scan over a block of NT threads.  s_w: NT / 64 words of LDS.
+template <int NT = kBlock>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, uint32_t &total)
 {
     const uint32_t inc = wave_incl_scan(v);
@@ -53,7 +54,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, u
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < kBlock / kWave; ++k) {
+    for (int k = 0; k < NT / kWave; ++k) {
         uint32_t t = s_w[k];
         off += (k < w) ? t : 0u;
         tot += t;
@@ -63,7 +64,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, u
     return off + inc - v;
 }
 
-// Sum over a 256-thread block; every thread gets the total.  s_w: 4 words of LDS.
+// Sum over a block of NT threads; every thread gets the total.  s_w: NT / 64 words of LDS.
+template <int NT = kBlock>
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v,
                                                             unsigned long long *s_w)
 {
@@ -74,7 +76,7 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     __syncthreads();
     unsigned long long tot = 0;
 #pragma unroll
-    for (int k = 0; k < kBlock / kWave; ++k) tot += s_w[k];
+    for (int k = 0; k < NT / kWave; ++k) tot += s_w[k];
     __syncthreads();
     return tot;
 }
@@ -341,6 +343,49 @@ __global__ __launch_bounds__(kBlock) void k_scan2_down(ScanJob j0, ScanJob j1, i
             off += v[k];
             if (j.incl) j.incl[i0 + k] = off;
         }
+    }
+}
+
+// Both jobs in one launch when they are small (at most kSmallScan entries):
+// workgroup j scans job j alone, kScanItems entries per thread per pass,
+// carrying the running sum -- one launch instead of three, for arrays whose
+// scan time is launch latency.  Same outputs as the three-launch scan.
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallScan = 8192;            // at most two passes of the workgroup
+
+__global__ __launch_bounds__(kSmallBlock) void k_scan_small(ScanJob j0, ScanJob j1)
+{
+    __shared__ uint32_t s_w[kSmallBlock / kWave];
+    __shared__ unsigned long long s_q[kSmallBlock / kWave];
+    const ScanJob j = blockIdx.x == 0 ? j0 : j1;
+    uint32_t carry = 0;
+    unsigned long long q = 0;
+    for (int base = 0; base < j.n; base += kSmallBlock * kScanItems) {
+        const int i0 = base + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (i0 + k < j.n) ? j.in[i0 + k] : 0u;
+            s += v[k];
+            q += (unsigned long long)v[k] * v[k];
+        }
+        uint32_t total;
+        uint32_t off = block_excl_scan<kSmallBlock>(s, s_w, total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < j.n) {
+                if (j.zero_in) j.in[i0 + k] = 0u;
+                j.excl[i0 + k] = off;
+                off += v[k];
+                if (j.incl) j.incl[i0 + k] = off;
+            }
+        }
+        carry += total;
+    }
+    if (j.sq) {                                    // uniform per block
+        q = block_sum_u64<kSmallBlock>(q, s_q);
+        if (threadIdx.x == 0 && q) atomicAdd(j.sq, q);
     }
 }
 
