@@ -710,6 +710,11 @@ int scan_jobs(plife_physics *h, ScanJob a, ScanJob b)
     b.bsum = h->bsum + nb0;
     a.sqb = h->bsq;
     k_scan2_reduce<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
+    if (nb0 <= kFoldBlocks && nb1 <= kFoldBlocks) {
+        k_scan2_down_fold<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
+        LAUNCH_CHECK();
+        return PLIFE_OK;
+    }
     k_scan2_single<<<nb1 ? 2 : 1, kBlock, 0, h->stream>>>(a, b);
     k_scan2_down<<<nb0 + nb1, kBlock, 0, h->stream>>>(a, b, nb0);
     LAUNCH_CHECK();

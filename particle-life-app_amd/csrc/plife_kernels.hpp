@@ -346,6 +346,51 @@ __global__ __launch_bounds__(kBlock) void k_scan2_down(ScanJob j0, ScanJob j1, i
     }
 }
 
+// Down-sweep for jobs of at most kFoldBlocks blocks, with the block-sum scan
+// folded in: each block adds up the sums of the blocks before it (at most
+// kFoldBlocks / kBlock loads per thread), and block 0 adds the job's sum of
+// squares -- two launches instead of three, same outputs.
+constexpr int kFoldBlocks = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_scan2_down_fold(ScanJob j0, ScanJob j1, int nb0)
+{
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_q[kBlock / kWave];
+    ScanJob j;
+    int blk;
+    scan_job_of(j0, j1, nb0, blockIdx.x, j, blk);
+    uint32_t pre = 0;
+    for (int b = threadIdx.x; b < blk; b += kBlock) pre += j.bsum[b];
+    uint32_t before;
+    block_excl_scan(pre, s_w, before);             // `before` = the sum over all threads
+    const int i0 = blk * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (i0 + k < j.n) ? j.in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(s, s_w, total) + before;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (i0 + k < j.n) {
+            if (j.zero_in) j.in[i0 + k] = 0u;
+            j.excl[i0 + k] = off;
+            off += v[k];
+            if (j.incl) j.incl[i0 + k] = off;
+        }
+    }
+    if (j.sq && blk == 0) {                        // uniform per block
+        const int nb = (j.n + kScanTile - 1) / kScanTile;
+        unsigned long long q = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) q += j.sqb[b];
+        q = block_sum_u64(q, s_q);
+        if (threadIdx.x == 0 && q) atomicAdd(j.sq, q);
+    }
+}
+
 // Both jobs in one launch when they are small (at most kSmallScan entries):
 // workgroup j scans job j alone, kScanItems entries per thread per pass,
 // carrying the running sum -- one launch instead of three, for arrays whose
