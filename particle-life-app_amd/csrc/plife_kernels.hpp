@@ -24,7 +24,7 @@ namespace plife {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kRadixTile = 4096;                 // elements per radix tile
+constexpr int kRadixTile = 2048;                 // elements per radix tile
 constexpr int kScatterBlock = 512;               // threads of the radix scatter
 constexpr int kRadixItems = kRadixTile / kScatterBlock;
 constexpr double kFix = 4294967296.0;       // 2^32
