@@ -26,6 +26,7 @@ def _mixed_worlds(dim=2):
         w(206, 800, 7, rmax=0.4, dt=0.01, dim=dim),                        # nx = 2 (literal stencil)
         worlds.gaussian_blob_world(207, 9_000, 5, rmax=0.03, dt=0.01, beta=0.3, nblobs=3,
                                    sigma=0.02, dim=dim),
+        w(208, 4_000, 40, rmax=0.03, dt=0.01, dim=dim),   # > 32 types: the batch's matrices in global memory
     ]
     return out
 

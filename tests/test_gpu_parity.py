@@ -38,6 +38,10 @@ CASES = {
     "3d": (109, 20_000, 6, 0.06, 0.01, {"dim": 3}),
     "3d_clamped": (110, 8_000, 5, 0.09, 0.01, {"dim": 3, "wrap": False}),
     "3d_nx2": (111, 300, 3, 0.45, 0.01, {"dim": 3}),
+    # matrices too large for LDS (> 32 types): read from global memory
+    "many_types_2d": (140, 8_000, 40, 0.03, 0.02, {}),
+    "max_types_2d": (141, 20_000, 256, 0.02, 0.02, {}),
+    "many_types_3d": (142, 20_000, 64, 0.06, 0.01, {"dim": 3}),
 }
 
 
@@ -171,6 +175,7 @@ SPARSE = {
     "sparse_3d_clamped": (131, 20_000, 5, 0.03, 3, False),  # 0.55
     "sparse_2d": (132, 3_000, 4, 0.01, 2, True),           # 0.30
     "sparse_2d_clamped": (133, 2_000, 4, 0.015, 2, False),  # 0.46
+    "sparse_3d_48_types": (134, 20_000, 48, 0.02, 3, True),  # matrix in global memory
 }
 
 
