@@ -184,3 +184,41 @@ def test_work_balanced_launches_equal_worlds_alone():
                 np.testing.assert_array_equal(g[key], s[key], err_msg=key)
     finally:
         e.close()
+
+
+def test_many_tiny_worlds():
+    """1,200 tiny worlds (0-60 particles, 1-4 types, rmax up to 0.5 so some
+    grids have fewer than 3 cells per axis) in one batch: a sample of them
+    equals the same world stepped alone, and pair counts add up."""
+    from particle_life import Batch, Engine, worlds
+
+    print("\n[batch] case=many_tiny_worlds", flush=True)
+    rng = np.random.default_rng(11)
+    ws = []
+    for k in range(1200):
+        n = int(rng.choice([0, 1, 2, 7, 60]))
+        ws.append(worlds.uniform_world(600 + k, n, int(rng.integers(1, 5)),
+                                       rmax=float(rng.uniform(0.05, 0.5)), dt=0.01,
+                                       wrap=bool(k % 3)))
+    b = Batch(0)
+    try:
+        ids = [b.add_world(w) for w in ws]
+        for _ in range(3):
+            b.step(1)
+            b.sync()
+        sample = list(range(0, 1200, 41))
+        got = {k: b.state(ids[k]) for k in sample}
+        info = {k: b.info(ids[k]) for k in sample}
+    finally:
+        b.close()
+    e = Engine(0)
+    try:
+        for k in sample:
+            e.load_world(ws[k])
+            e.step(3)
+            s = e.state()
+            for key in ("pos", "vel", "types", "ids"):
+                np.testing.assert_array_equal(got[k][key], s[key], err_msg=f"world {k}: {key}")
+            assert info[k]["pairs"] == e.pair_count()
+    finally:
+        e.close()
