@@ -205,9 +205,8 @@ struct plife_physics {
     uint64_t blocks_serial = 0;        // bumped whenever the table's content or order changes
     std::vector<uint64_t> block_key;   // per entry: world id, sparse kernel, first home in the world
     std::vector<uint32_t> block_est;   // per entry: expected pairs (before a step measures them)
-    uint32_t *d_block_work = nullptr;  // pairs of each table entry in the last step, + tag
-    size_t block_work_cap = 0;
-    uint32_t *work_host = nullptr;     // pinned copy of d_block_work
+    uint32_t *work_host = nullptr;     // pinned: each block's pairs, written by the force kernels
+    uint32_t *work_dev = nullptr;      // the device's address of work_host
     size_t work_host_cap = 0;
     uint64_t classes = 0;              // bumped whenever a world changes force kernel
 
@@ -641,8 +640,6 @@ int ensure_blocks(plife_physics *h, int bs)
     h->nb_total = (int)h->blocks.size();
     int rc;
     if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size(), &h->retired))) return rc;
-    if ((rc = ensure_dev(&h->d_block_work, &h->block_work_cap, h->blocks.size() + 1, &h->retired)))
-        return rc;
     if (h->blocks.size() + 1 > h->work_host_cap) {
         // the old buffer may still be written by queued steps: keep it until destroy
         const size_t cap = grown(h->work_host_cap, h->blocks.size() + 1);
@@ -650,8 +647,15 @@ int ensure_blocks(plife_physics *h, int bs)
         const hipError_t e = hipHostMalloc((void **)&p, cap * 4, hipHostMallocDefault);
         if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
         std::memset(p, 0, cap * 4);
+        uint32_t *dp = nullptr;
+        const hipError_t e2 = hipHostGetDevicePointer((void **)&dp, p, 0);
+        if (e2 != hipSuccess) {
+            (void)hipHostFree(p);
+            return fail(PLIFE_ERR_DEVICE, "hipHostGetDevicePointer: %s", hipGetErrorString(e2));
+        }
         if (h->work_host) h->retired_host.push_back(h->work_host);
         h->work_host = p;
+        h->work_dev = dp;
         h->work_host_cap = cap;
     }
     ++h->blocks_serial;
@@ -1191,7 +1195,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.max_cells = fp.max_cells;
     a.pair_total = h->d_pairs;
     a.pair_world = h->d_pairs + 2;
-    a.block_work = h->d_block_work;
+    a.block_work = h->work_dev;
     a.work_tag = (uint32_t)(h->blocks_serial + 1);
     a.nb_total = h->nb_total;
     a.dbg_count = h->dbg_count;
@@ -1204,8 +1208,6 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     // the step's counters (pairs, clustering measure, per-world pairs, layout
     // tag) travel to pinned host memory for the launch planner
     HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs, (3 + h->worlds.size()) * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->work_host, h->d_block_work, ((size_t)h->nb_total + 1) * 4,
                            hipMemcpyDeviceToHost, h->stream));
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
@@ -1242,7 +1244,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
                        h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash,
                        h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks,
-                       h->stat_host, h->d_block_work, h->work_host};
+                       h->stat_host, h->work_host};
     std::memcpy(k.ptr, p, sizeof p);
     std::vector<unsigned char> v(sizeof k);
     std::memcpy(v.data(), &k, sizeof k);
@@ -1702,7 +1704,6 @@ void destroy_handle(plife_physics *h)
     dev_free(h->world_of);
     dev_free(h->d_src_off);
     dev_free(h->d_blocks);
-    dev_free(h->d_block_work);
     dev_free(h->d_edits);
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);

@@ -49,7 +49,7 @@ struct ForceArgs {
     int cap, lcap, max_cells;
     unsigned long long *pair_total;  // interacting pairs of the step, all worlds
     unsigned long long *pair_world;  // per world
-    uint32_t *block_work;            // per block-table entry: its pairs (saturated); [nb_total] = tag
+    uint32_t *block_work;            // pinned host memory: per table entry its pairs (saturated); [nb_total] = tag
     uint32_t work_tag;               // identifies the block table the counts belong to
     int nb_total;
     int32_t *dbg_count;
