@@ -1,7 +1,7 @@
 """Pins the FP64 oracle (oracle/plife_oracle.c) without running the reference.
 
-The reference has no tests or fixtures (SURVEY: none in src/), and its Java
-cannot run here, so every check below comes from something the oracle's code
+The reference has no tests or fixtures (nothing under src/test or among its
+resources), and its Java cannot run here, so every check below comes from something the oracle's code
 does not share: closed forms of the force law, hand-solved two-particle cases,
 library routines (numpy's stable argsort / bincount), an independent numpy
 all-pairs predicate, brute force, and invariants of the dynamics.
