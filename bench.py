@@ -339,7 +339,8 @@ def main() -> int:
                             "ids as double[3N]/int[N] per frame, wall clock"},
         "roofline": {
             "kernel": f"k_force<{w.dim}, wrap, no-debug>",
-            "bound": "valu",
+            "bound": "mfma",            # the compute roofline (FP32 peak), not HBM
+            "pipe": "valu: FP32 screen + FP64 force law as vector instructions; no matrix ops",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
