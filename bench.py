@@ -254,6 +254,8 @@ def main() -> int:
     if args.config == "c5":
         return bench_serve(args)
 
+    import numpy as np
+
     from particle_life import Engine, worlds
 
     make = {"c1": worlds.config1, "c2": worlds.config2, "c3": worlds.config3}[args.config]
@@ -274,21 +276,24 @@ def main() -> int:
     # per-phase timing (events around each phase) on further steps
     prof = eng.profile(args.profile_steps)
 
-    # the app's path: one update and a full state download (PhysicsSnapshot)
-    # per frame, host-visible wall time
+    # the app's path: one update and the snapshot's download per frame
+    # (PhysicsSnapshot.write: positions, velocities, types into arrays kept
+    # across frames), host-visible wall time
     frames = 20
-    for _ in range(2):          # the first download allocates the pinned staging buffer
+    snap_pos, snap_vel = np.empty((w.n, 3)), np.empty((w.n, 3))
+    snap_typ = np.empty(w.n, np.int32)
+    for _ in range(2):          # the first download allocates the staging buffers
         eng.step(1)
-        eng.state()
+        eng.download(snap_pos, snap_vel, snap_typ)
     eng.sync()
     t0 = time.perf_counter()
     for _ in range(frames):
         eng.step(1)
-        eng.state()
+        eng.download(snap_pos, snap_vel, snap_typ)
     frame_ms = (time.perf_counter() - t0) * 1e3 / frames
     t0 = time.perf_counter()
     for _ in range(frames):
-        eng.state()
+        eng.download(snap_pos, snap_vel, snap_typ)
     get_state_ms = (time.perf_counter() - t0) * 1e3 / frames
     pairs = prof["pairs"]
     steps_per_s = args.steps / (ms / 1e3)
@@ -335,8 +340,9 @@ def main() -> int:
         },
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
         "app_path": {"ms_per_frame": frame_ms, "get_state_ms": get_state_ms,
-                     "how": "update(1) + plife_get_particles of positions, velocities, types and "
-                            "ids as double[3N]/int[N] per frame, wall clock"},
+                     "how": "update(1) + plife_get_particles of positions, velocities and types "
+                            "into double[3N]/int[N] arrays reused across frames (as "
+                            "PhysicsSnapshot.write), wall clock"},
         "roofline": {
             "kernel": f"k_force<{w.dim}, wrap, no-debug>",
             "bound": "mfma",            # the compute roofline (FP32 peak), not HBM

@@ -85,8 +85,13 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
                         const int32_t *types);
 
 /* Downloads the state in the current device array order (cell-sorted after
- * a step, like Physics.particles after makeContainers' swap).  Any output
- * pointer may be NULL.  Synchronises the handle's stream. */
+ * a step, like Physics.particles after makeContainers' swap), for the
+ * snapshot (app/PhysicsSnapshot.java:37-61) or Physics.particles.  Any output
+ * pointer may be NULL; only the arrays asked for cross PCIe.  Synchronises the
+ * handle's stream. */
+int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
+                        int32_t *ids);
+
 /* In-place edits between steps -- the app's cursor tools change a few
  * particles (app/Main.java:543-580) -- without re-uploading the rest: the
  * particle at slots[i] of the current order (the order plife_get_particles
@@ -95,9 +100,6 @@ int plife_set_particles(plife_physics *h, int32_t n, const double *pos, const do
  * and in [0, n), types in [0, ntypes). */
 int plife_set_particle_slots(plife_physics *h, int32_t k, const int32_t *slots, const double *pos,
                              const double *vel, const int32_t *types);
-
-int plife_get_particles(plife_physics *h, int32_t n, double *pos, double *vel, int32_t *types,
-                        int32_t *ids);
 
 int plife_get_count(plife_physics *h, int32_t *n);
 

@@ -23,6 +23,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <sched.h>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -115,6 +116,8 @@ struct UploadSlot {
     hipEvent_t done = nullptr;
 };
 constexpr int kUploadSlots = 4;
+constexpr int kDlChunks = 8;           // downloads move in at most this many chunks
+constexpr int kDlMinChunk = 65536;     // slots; smaller downloads are one chunk
 
 // Grow-only device array: reallocated (contents dropped) when too small.  The
 // old array goes to `retired` when given and small enough, else is freed.
@@ -236,6 +239,9 @@ struct plife_physics {
     std::vector<void *> retired_host;  // replaced pinned buffers, freed at destroy
     unsigned char *pin = nullptr;      // pinned staging of state downloads
     size_t pin_cap = 0;
+    uint32_t *dl_dev = nullptr;        // device staging of state downloads (chunk-major)
+    size_t dl_cap = 0;
+    hipEvent_t dl_ev[kDlChunks] = {};  // one per download chunk: its DMA has completed
     UploadSlot up[kUploadSlots];       // pinned staging of table uploads, used in turn
     int up_cur = 0;
     size_t up_used = 0;
@@ -1587,80 +1593,133 @@ int set_slots(plife_physics *h, WorldHost &w, int32_t k, const int32_t *slots, c
     return PLIFE_OK;
 }
 
-// Runs f(lo, hi) over [0, n) on up to 8 host threads (one for small n).
-template <typename F>
-void parallel_chunks(int n, F f)
+// Host threads for conversions: the CPUs this process may run on, at most 16.
+int host_threads()
 {
-    const int nt = n >= (1 << 16) ? (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency())) : 1;
-    if (nt <= 1) {
-        f(0, n);
-        return;
-    }
-    std::vector<std::thread> ts;
-    const int chunk = (n + nt - 1) / nt;
-    for (int t = 0; t < nt; ++t) {
-        const int lo = t * chunk, hi = std::min(n, lo + chunk);
-        if (lo < hi) ts.emplace_back(f, lo, hi);
-    }
-    for (auto &t : ts) t.join();
+    cpu_set_t s;
+    int n = 0;
+    if (sched_getaffinity(0, sizeof(s), &s) == 0) n = CPU_COUNT(&s);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 16));
 }
 
-// Downloads slots [off, off + n) of the current buffers: one DMA per array
-// into pinned staging, then the conversion to the caller's FP64 triples on
-// host threads.
+// Downloads slots [off, off + n) of the current buffers.  k_pack gathers the
+// requested arrays chunk by chunk into device staging, one DMA per chunk moves
+// it to pinned memory, and host threads convert each chunk to the caller's
+// FP64 triples as soon as its DMA has completed, while later chunks are still
+// in flight.
 int download_particles(plife_physics *h, int off, int n, double *pos, double *vel, int32_t *types,
                        int32_t *ids, bool d3)
 {
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (n == 0) return PLIFE_OK;
-    const size_t b = (size_t)n * 4;
-    const size_t need = 8 * b;
-    if (need > h->pin_cap) {
+    const PosBuf P = h->P(h->pc);
+    const AuxBuf A = h->A(h->ac);
+    PackArgs pk{};
+    int na = 0;
+    auto add = [&](bool want, const void *p) {
+        if (!want) return -1;
+        pk.src[na] = (const uint32_t *)p;
+        return na++;
+    };
+    const int ix = add(pos, P.x), iy = add(pos, P.y), iz = add(pos && d3, P.z);
+    const int ivx = add(vel, A.vx), ivy = add(vel, A.vy), ivz = add(vel && d3, A.vz);
+    const int it = add(types, A.type), ii = add(ids, A.id);
+    if (na == 0) return PLIFE_OK;
+    const size_t words = (size_t)na * (size_t)n;
+    int rc = ensure_dev(&h->dl_dev, &h->dl_cap, words);
+    if (rc) return rc;
+    if (words * 4 > h->pin_cap) {
         if (h->pin) (void)hipHostFree(h->pin);
         h->pin = nullptr;
         h->pin_cap = 0;
-        const hipError_t e = hipHostMalloc((void **)&h->pin, need, hipHostMallocDefault);
+        const hipError_t e = hipHostMalloc((void **)&h->pin, words * 4, hipHostMallocDefault);
         if (e != hipSuccess) {
             h->pin = nullptr;
-            return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", need, hipGetErrorString(e));
+            return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", words * 4,
+                        hipGetErrorString(e));
         }
-        h->pin_cap = need;
+        h->pin_cap = words * 4;
     }
-    const PosBuf P = h->P(h->pc);
-    const AuxBuf A = h->A(h->ac);
-    uint32_t *sx = (uint32_t *)h->pin, *sy = sx + n, *sz = sy + n;
-    float *svx = (float *)(sz + n), *svy = svx + n, *svz = svy + n;
-    int32_t *st = (int32_t *)(svz + n), *sid = st + n;
-    const hipMemcpyKind k = hipMemcpyDeviceToHost;
-    if (pos) {
-        HIP_TRY(hipMemcpyAsync(sx, P.x + off, b, k, h->stream));
-        HIP_TRY(hipMemcpyAsync(sy, P.y + off, b, k, h->stream));
-        if (d3) HIP_TRY(hipMemcpyAsync(sz, P.z + off, b, k, h->stream));
+    const int chunk = n >= 2 * kDlMinChunk
+                          ? (int)std::min<long long>(n, ((long long)ceil_div(n, kDlChunks) + 255) / 256 * 256)
+                          : n;
+    const int nchunks = ceil_div(n, chunk);
+    pk.na = na;
+    pk.off = off;
+    pk.n = n;
+    pk.chunk = chunk;
+    k_pack<<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(pk, h->dl_dev);
+    LAUNCH_CHECK();
+    uint32_t *const stage = (uint32_t *)h->pin;
+    for (int c = 0; c < nchunks; ++c) {
+        const size_t first = (size_t)c * chunk * na;
+        const size_t len = (size_t)std::min(chunk, n - c * chunk);
+        HIP_TRY(hipMemcpyAsync(stage + first, h->dl_dev + first, len * na * 4, hipMemcpyDeviceToHost,
+                               h->stream));
+        HIP_TRY(hipEventRecord(h->dl_ev[c], h->stream));
     }
-    if (vel) {
-        HIP_TRY(hipMemcpyAsync(svx, A.vx + off, b, k, h->stream));
-        HIP_TRY(hipMemcpyAsync(svy, A.vy + off, b, k, h->stream));
-        if (d3) HIP_TRY(hipMemcpyAsync(svz, A.vz + off, b, k, h->stream));
-    }
-    if (types) HIP_TRY(hipMemcpyAsync(st, A.type + off, b, k, h->stream));
-    if (ids) HIP_TRY(hipMemcpyAsync(sid, A.id + off, b, k, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    parallel_chunks(n, [&](int lo, int hi) {
-        for (int i = lo; i < hi; ++i) {
-            if (pos) {
-                pos[3 * i] = (double)sx[i] * kInvFix;
-                pos[3 * i + 1] = (double)sy[i] * kInvFix;
-                pos[3 * i + 2] = d3 ? (double)sz[i] * kInvFix : 0.0;
-            }
-            if (vel) {
-                vel[3 * i] = svx[i];
-                vel[3 * i + 1] = svy[i];
-                vel[3 * i + 2] = d3 ? (double)svz[i] : 0.0;
+
+    // chunk c, its slots [lo, hi) (relative to the chunk)
+    auto convert = [&](int c, int lo, int hi) {
+        const int g = c * chunk;
+        const int len = std::min(chunk, n - g);
+        const uint32_t *base = stage + (size_t)g * na;
+        auto col = [&](int a) { return base + (size_t)a * len; };
+        if (pos) {
+            const uint32_t *x = col(ix), *y = col(iy), *z = d3 ? col(iz) : nullptr;
+            for (int i = lo; i < hi; ++i) {
+                double *o = pos + 3 * (size_t)(g + i);
+                o[0] = (double)x[i] * kInvFix;
+                o[1] = (double)y[i] * kInvFix;
+                o[2] = z ? (double)z[i] * kInvFix : 0.0;
             }
         }
-        if (types) std::memcpy(types + lo, st + lo, (size_t)(hi - lo) * 4);
-        if (ids) std::memcpy(ids + lo, sid + lo, (size_t)(hi - lo) * 4);
-    });
+        if (vel) {
+            const float *x = (const float *)col(ivx), *y = (const float *)col(ivy);
+            const float *z = d3 ? (const float *)col(ivz) : nullptr;
+            for (int i = lo; i < hi; ++i) {
+                double *o = vel + 3 * (size_t)(g + i);
+                o[0] = x[i];
+                o[1] = y[i];
+                o[2] = z ? (double)z[i] : 0.0;
+            }
+        }
+        if (types) std::memcpy(types + g + lo, col(it) + lo, (size_t)(hi - lo) * 4);
+        if (ids) std::memcpy(ids + g + lo, col(ii) + lo, (size_t)(hi - lo) * 4);
+    };
+    // This thread waits for the chunks' events in order and publishes them;
+    // every thread (this one included) converts its share of each chunk.
+    const int nt = n >= kDlMinChunk ? host_threads() : 1;
+    std::atomic<int> ready{0};
+    std::atomic<bool> failed{false};
+    auto share = [&](int t, int c) {
+        const int len = std::min(chunk, n - c * chunk);
+        const long long lo = (long long)len * t / nt, hi = (long long)len * (t + 1) / nt;
+        if (lo < hi) convert(c, (int)lo, (int)hi);
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < nt; ++t)
+        ts.emplace_back([&, t]() {
+            for (int c = 0; c < nchunks; ++c) {
+                while (ready.load(std::memory_order_acquire) <= c) std::this_thread::yield();
+                if (failed.load(std::memory_order_acquire)) return;
+                share(t, c);
+            }
+        });
+    hipError_t err = hipSuccess;
+    for (int c = 0; c < nchunks; ++c) {
+        err = hipEventSynchronize(h->dl_ev[c]);
+        if (err != hipSuccess) {
+            failed.store(true, std::memory_order_release);
+            ready.store(nchunks, std::memory_order_release);
+            break;
+        }
+        ready.store(c + 1, std::memory_order_release);
+        share(0, c);
+    }
+    for (auto &t : ts) t.join();
+    if (err != hipSuccess) return fail(PLIFE_ERR_DEVICE, "download: %s", hipGetErrorString(err));
     return PLIFE_OK;
 }
 
@@ -1677,6 +1736,10 @@ int create_handle(int32_t device, plife_physics *h)
     if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     for (UploadSlot &s : h->up) {
         e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    for (hipEvent_t &ev : h->dl_ev) {
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
     }
     e = hipHostMalloc((void **)&h->stat_host, 4 * sizeof(uint64_t), hipHostMallocDefault);
@@ -1712,6 +1775,11 @@ void destroy_handle(plife_physics *h)
     for (void *p : h->retired_host) (void)hipHostFree(p);
     h->retired_host.clear();
     if (h->pin) (void)hipHostFree(h->pin);
+    dev_free(h->dl_dev);
+    for (hipEvent_t &ev : h->dl_ev) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
     for (void *p : h->retired) (void)hipFree(p);
     h->retired.clear();
     for (UploadSlot &s : h->up) {

@@ -878,4 +878,25 @@ __global__ __launch_bounds__(kScatterBlock) void k_radix_scatter(const uint32_t 
     }
 }
 
+// Slots [off, off + n) of up to kPackMax device arrays of 32-bit words,
+// gathered chunk-major for a chunked download: chunk c (slots c*chunk ...)
+// holds each array's len_c words in turn, so one DMA moves one chunk.
+constexpr int kPackMax = 8;
+struct PackArgs {
+    const uint32_t *src[kPackMax];
+    int na, off, n, chunk;
+};
+
+__global__ __launch_bounds__(kBlock) void k_pack(PackArgs p, uint32_t *dst)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const int c = i / p.chunk, j = i - c * p.chunk;
+    const int len = min(p.chunk, p.n - c * p.chunk);
+    uint32_t *d = dst + (size_t)c * p.chunk * p.na + j;
+#pragma unroll
+    for (int a = 0; a < kPackMax; ++a)
+        if (a < p.na) d[(size_t)a * len] = p.src[a][p.off + i];
+}
+
 }  // namespace plife

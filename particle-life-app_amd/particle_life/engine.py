@@ -105,9 +105,25 @@ class Engine:
         vel = np.empty((n, 3))
         typ = np.empty(n, np.int32)
         ids = np.empty(n, np.int32)
-        check(self._lib.plife_get_particles(self._h, n, pos.ctypes.data, vel.ctypes.data,
-                                            typ.ctypes.data, ids.ctypes.data))
+        self.download(pos, vel, typ, ids)
         return {"pos": pos, "vel": vel, "types": typ, "ids": ids}
+
+    def download(self, pos=None, vel=None, types=None, ids=None) -> None:
+        """plife_get_particles into caller-owned arrays (None = not wanted):
+        pos/vel C-contiguous float64 (n, 3), types/ids C-contiguous int32 (n,).
+        Reusing the arrays across frames saves the page faults of fresh ones."""
+        n = self.n
+        ptrs = []
+        for a, shape, dt in ((pos, (n, 3), np.float64), (vel, (n, 3), np.float64),
+                             (types, (n,), np.int32), (ids, (n,), np.int32)):
+            if a is None:
+                ptrs.append(None)
+                continue
+            if a.shape != shape or a.dtype != dt or not a.flags.c_contiguous or not a.flags.writeable:
+                raise ValueError(f"download needs a writeable C-contiguous {np.dtype(dt).name} "
+                                 f"array of shape {shape}, got {a.dtype} {a.shape}")
+            ptrs.append(a.ctypes.data)
+        check(self._lib.plife_get_particles(self._h, n, *ptrs))
 
     # stepping
     def step(self, nsteps: int = 1) -> int:

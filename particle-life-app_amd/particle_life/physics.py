@@ -15,6 +15,7 @@ host view may have been mutated and downloaded lazily after steps.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass
 from typing import Callable, Optional
 
@@ -218,13 +219,22 @@ class Particles:
 class PhysicsSnapshot:
     """app/PhysicsSnapshot.java:11-62: double[3N] positions/velocities + types."""
 
-    def __init__(self, positions, velocities, types, settings, particleCount, typeCount):
+    def __init__(self, positions=None, velocities=None, types=None, settings=None,
+                 particleCount: int = 0, typeCount=None):
         self.positions = positions
         self.velocities = velocities
         self.types = types
         self.settings = settings
         self.particleCount = particleCount
         self.typeCount = typeCount
+        self.snapshotTime = 0
+
+    def take(self, physics: "GpuPhysics") -> None:
+        """PhysicsSnapshot.take (app/PhysicsSnapshot.java:24-34): rewrites this
+        snapshot's arrays in place, re-allocated only when the particle count
+        changed (PhysicsSnapshot.java:39-44), straight from the device when it
+        holds the newer state; then the settings, type counts and time stamp."""
+        physics._write_snapshot(self)
 
 
 # ---------------------------------------------------------------------------
@@ -302,12 +312,30 @@ class GpuPhysics:
         self._host_dirty = True
 
     def snapshot(self) -> PhysicsSnapshot:
-        """PhysicsSnapshot.take (app/PhysicsSnapshot.java:24-34): read-only copy."""
-        self._download()
-        p = self._particles
-        counts = np.bincount(p.type, minlength=self.settings.matrix.size())
-        return PhysicsSnapshot(p.position.reshape(-1).copy(), p.velocity.reshape(-1).copy(),
-                               p.type.copy(), self.settings.deepCopy(), len(p), counts)
+        """A new PhysicsSnapshot taken of this physics (PhysicsSnapshot.take)."""
+        s = PhysicsSnapshot()
+        s.take(self)
+        return s
+
+    def _write_snapshot(self, snap: PhysicsSnapshot) -> None:
+        n = self._engine.n if self._device_newer else len(self._particles)
+        if snap.types is None or len(snap.types) != n:
+            snap.positions = np.empty(3 * n)
+            snap.velocities = np.empty(3 * n)
+            snap.types = np.empty(n, np.int32)
+        if self._device_newer:
+            # positions, velocities and types only: no ids, no host view refresh
+            self._engine.download(snap.positions.reshape(n, 3), snap.velocities.reshape(n, 3),
+                                  snap.types)
+        else:
+            p = self._particles
+            snap.positions[:] = p.position.reshape(-1)
+            snap.velocities[:] = p.velocity.reshape(-1)
+            snap.types[:] = p.type
+        snap.settings = self.settings.deepCopy()
+        snap.particleCount = n
+        snap.typeCount = np.bincount(snap.types, minlength=self.settings.matrix.size())
+        snap.snapshotTime = int(time.time() * 1000)
 
     def editParticles(self, indices, position=None, velocity=None, type_=None) -> None:
         """In-place edits of some particles between steps (the app's cursor

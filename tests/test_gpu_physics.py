@@ -209,3 +209,86 @@ def test_batch_slot_edits_and_gpu_physics_edits():
         phys.update(2)
     finally:
         phys.kill()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 300_001), (3, 140_000), (2, 70_000)])
+def test_chunked_download_returns_the_uploaded_state(dim, n):
+    """Downloads of 2 x 65,536 particles or more move in up to 8 chunks that
+    host threads convert while later chunks are still in flight.  Every array,
+    requested alone or with the others, into fresh or reused arrays, equals
+    the uploaded state: positions on the 2^-24 grid and FP32-representable
+    velocities pass through the device exactly.  Also for a world that is not
+    the first of a batch (its slots start past the other world's)."""
+    from particle_life import Batch, Engine, worlds
+
+    print(f"\n[physics] case=chunked_download dim={dim} n={n}", flush=True)
+    w = worlds.uniform_world(31 + dim, n, 5, rmax=0.01, dt=0.01, dim=dim)
+    rng = np.random.default_rng(dim)
+    w.vel = rng.uniform(-1e-3, 1e-3, (n, 3)).astype(np.float32).astype(np.float64)
+    if dim == 2:
+        w.vel[:, 2] = 0.0
+
+    def check_against_upload(st):
+        np.testing.assert_array_equal(np.sort(st["ids"]), np.arange(n))
+        o = np.argsort(st["ids"])
+        np.testing.assert_array_equal(st["pos"][o], w.pos)
+        np.testing.assert_array_equal(st["vel"][o], w.vel)
+        np.testing.assert_array_equal(st["types"][o], w.types)
+
+    e = Engine(0)
+    e.load_world(w)
+    full = e.state()
+    check_against_upload(full)
+    pos, vel = np.full((n, 3), np.nan), np.full((n, 3), np.nan)
+    typ, ids = np.full(n, -1, np.int32), np.full(n, -1, np.int32)
+    e.download(pos=pos)
+    np.testing.assert_array_equal(pos, full["pos"])
+    e.download(vel=vel)
+    np.testing.assert_array_equal(vel, full["vel"])
+    e.download(types=typ)
+    np.testing.assert_array_equal(typ, full["types"])
+    e.download(ids=ids)
+    np.testing.assert_array_equal(ids, full["ids"])
+    # the same arrays again after steps, against a fresh full download
+    e.step(2)
+    e.download(pos, vel, typ)
+    after = e.state()
+    np.testing.assert_array_equal(pos, after["pos"])
+    np.testing.assert_array_equal(vel, after["vel"])
+    np.testing.assert_array_equal(typ, after["types"])
+    with pytest.raises(ValueError):
+        e.download(pos=np.empty((n, 3), np.float32))
+    e.close()
+
+    with Batch(0) as b:
+        small = worlds.uniform_world(7, 1000, 3, rmax=0.05, dt=0.01, dim=dim)
+        b.add_world(small)
+        wid = b.add_world(w)
+        check_against_upload(b.state(wid))
+
+
+def test_snapshot_take_reuses_its_arrays():
+    """PhysicsSnapshot.take rewrites the snapshot's arrays in place while the
+    particle count stays (app/PhysicsSnapshot.java:39-44), new ones when it
+    changes; the values are those of the physics' particles."""
+    from particle_life import PhysicsSnapshot
+
+    print("\n[physics] case=snapshot_take", flush=True)
+    ph = _physics(2000)
+    ph.update(2)
+    s = PhysicsSnapshot()
+    s.take(ph)
+    arrays = (s.positions, s.velocities, s.types)
+    ph.update(1)
+    s.take(ph)
+    assert all(a is b for a, b in zip(arrays, (s.positions, s.velocities, s.types)))
+    p = ph.particles                              # host view of the same state
+    np.testing.assert_array_equal(s.positions, p.position.reshape(-1))
+    np.testing.assert_array_equal(s.velocities, p.velocity.reshape(-1))
+    np.testing.assert_array_equal(s.types, p.type)
+    assert s.particleCount == 2000 and s.typeCount.sum() == 2000 and s.snapshotTime > 0
+    ph.setParticleCount(1500)
+    s.take(ph)                                    # from the host view (not yet uploaded)
+    assert s.positions.shape == (4500,) and s.positions is not arrays[0]
+    np.testing.assert_array_equal(s.types, ph.particles.type)
+    ph.kill()
