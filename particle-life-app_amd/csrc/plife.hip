@@ -1448,35 +1448,94 @@ void store_settings(plife_physics *h, WorldHost &w, const plife_settings *s, int
     h->params_dirty = true;
 }
 
-// Converts n particles (Physics.ensurePosition, fixed point) and uploads them
-// to slots [off, off + n) of the current buffers, with ids 0..n-1.
+// Host threads for conversions: the CPUs this process may run on, at most 16.
+int host_threads()
+{
+    cpu_set_t s;
+    int n = 0;
+    if (sched_getaffinity(0, sizeof(s), &s) == 0) n = CPU_COUNT(&s);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 16));
+}
+
+// Runs f(lo, hi) over [0, n) on the host threads (one below 65,536 items).
+template <typename F>
+void parallel_for(int n, F f)
+{
+    const int nt = n >= kDlMinChunk ? host_threads() : 1;
+    std::vector<std::thread> ts;
+    for (int t = 1; t < nt; ++t)
+        ts.emplace_back([&, t]() {
+            const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+            if (lo < hi) f(lo, hi);
+        });
+    const int hi0 = (int)((long long)n / nt);
+    if (hi0 > 0) f(0, hi0);
+    for (auto &th : ts) th.join();
+}
+
+// The handle's pinned staging for up- and downloads, at least `bytes`.
+int ensure_pin(plife_physics *h, size_t bytes)
+{
+    if (bytes <= h->pin_cap) return PLIFE_OK;
+    if (h->pin) (void)hipHostFree(h->pin);
+    h->pin = nullptr;
+    h->pin_cap = 0;
+    const hipError_t e = hipHostMalloc((void **)&h->pin, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        h->pin = nullptr;
+        return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    h->pin_cap = bytes;
+    return PLIFE_OK;
+}
+
+// Converts n particles (Physics.ensurePosition, fixed point; FP32 velocities)
+// on the host threads into pinned staging, moves them with one DMA and
+// scatters them with k_unpack into slots [off, off + n) of the current
+// buffers, with ids 0..n-1.  The caller has synchronised the stream.
 int upload_particles(plife_physics *h, int off, int n, const double *pos, const double *vel,
                      const int32_t *types, bool wrap, bool d3)
 {
     if (n == 0) return PLIFE_OK;
-    std::vector<uint32_t> x(n), y(n), z(n);
-    std::vector<float> fx(n), fy(n), fz(n);
-    std::vector<int32_t> ids(n);
-    for (int i = 0; i < n; ++i) {
-        x[i] = fixed_of(pos[3 * i], wrap);
-        y[i] = fixed_of(pos[3 * i + 1], wrap);
-        z[i] = d3 ? fixed_of(pos[3 * i + 2], wrap) : 0u;
-        fx[i] = (float)vel[3 * i];
-        fy[i] = (float)vel[3 * i + 1];
-        fz[i] = d3 ? (float)vel[3 * i + 2] : 0.0f;
-        ids[i] = i;
-    }
-    const size_t b = (size_t)n * 4;
+    const int na = d3 ? 7 : 5;           // x, y, [z], vx, vy, [vz], type
+    const size_t words = (size_t)na * (size_t)n, nn = (size_t)n;
+    int rc = ensure_pin(h, words * 4);
+    if (rc) return rc;
+    if ((rc = ensure_dev(&h->dl_dev, &h->dl_cap, words))) return rc;
+    uint32_t *const st = (uint32_t *)h->pin;
+    uint32_t *const sv = st + (d3 ? 3 : 2) * nn;
+    parallel_for(n, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            st[i] = fixed_of(pos[3 * (size_t)i], wrap);
+            st[nn + i] = fixed_of(pos[3 * (size_t)i + 1], wrap);
+            if (d3) st[2 * nn + i] = fixed_of(pos[3 * (size_t)i + 2], wrap);
+            const float fx = (float)vel[3 * (size_t)i], fy = (float)vel[3 * (size_t)i + 1];
+            std::memcpy(sv + i, &fx, 4);
+            std::memcpy(sv + nn + i, &fy, 4);
+            if (d3) {
+                const float fz = (float)vel[3 * (size_t)i + 2];
+                std::memcpy(sv + 2 * nn + i, &fz, 4);
+            }
+        }
+        std::memcpy(sv + (d3 ? 3 : 2) * nn + lo, types + lo, (size_t)(hi - lo) * 4);
+    });
+    HIP_TRY(hipMemcpyAsync(h->dl_dev, st, words * 4, hipMemcpyHostToDevice, h->stream));
     const int p = h->pc, a = h->ac;
-    const hipMemcpyKind k = hipMemcpyHostToDevice;
-    HIP_TRY(hipMemcpyAsync(h->px[p] + off, x.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->py[p] + off, y.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->pz[p] + off, z.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->vx[a] + off, fx.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->vy[a] + off, fy.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->vz[a] + off, fz.data(), b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->type[a] + off, types, b, k, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->id[a] + off, ids.data(), b, k, h->stream));
+    UnpackArgs u{};
+    u.px = h->px[p];
+    u.py = h->py[p];
+    u.pz = h->pz[p];
+    u.vx = h->vx[a];
+    u.vy = h->vy[a];
+    u.vz = h->vz[a];
+    u.type = h->type[a];
+    u.id = h->id[a];
+    u.off = off;
+    u.n = n;
+    u.dim = d3 ? 3 : 2;
+    k_unpack<<<ceil_div(n, kBlock), kBlock, 0, h->stream>>>(u, h->dl_dev);
+    LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(h->stream));
     return PLIFE_OK;
 }
@@ -1593,16 +1652,6 @@ int set_slots(plife_physics *h, WorldHost &w, int32_t k, const int32_t *slots, c
     return PLIFE_OK;
 }
 
-// Host threads for conversions: the CPUs this process may run on, at most 16.
-int host_threads()
-{
-    cpu_set_t s;
-    int n = 0;
-    if (sched_getaffinity(0, sizeof(s), &s) == 0) n = CPU_COUNT(&s);
-    if (n <= 0) n = (int)std::thread::hardware_concurrency();
-    return std::max(1, std::min(n, 16));
-}
-
 // Downloads slots [off, off + n) of the current buffers.  k_pack gathers the
 // requested arrays chunk by chunk into device staging, one DMA per chunk moves
 // it to pinned memory, and host threads convert each chunk to the caller's
@@ -1629,18 +1678,7 @@ int download_particles(plife_physics *h, int off, int n, double *pos, double *ve
     const size_t words = (size_t)na * (size_t)n;
     int rc = ensure_dev(&h->dl_dev, &h->dl_cap, words);
     if (rc) return rc;
-    if (words * 4 > h->pin_cap) {
-        if (h->pin) (void)hipHostFree(h->pin);
-        h->pin = nullptr;
-        h->pin_cap = 0;
-        const hipError_t e = hipHostMalloc((void **)&h->pin, words * 4, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            h->pin = nullptr;
-            return fail(PLIFE_ERR_MEMORY, "hipHostMalloc of %zu bytes failed: %s", words * 4,
-                        hipGetErrorString(e));
-        }
-        h->pin_cap = words * 4;
-    }
+    if ((rc = ensure_pin(h, words * 4))) return rc;
     const int chunk = n >= 2 * kDlMinChunk
                           ? (int)std::min<long long>(n, ((long long)ceil_div(n, kDlChunks) + 255) / 256 * 256)
                           : n;

@@ -899,4 +899,32 @@ __global__ __launch_bounds__(kBlock) void k_pack(PackArgs p, uint32_t *dst)
         if (a < p.na) d[(size_t)a * len] = p.src[a][p.off + i];
 }
 
+// The inverse for uploads: staged arrays x, y, [z], vx, vy, [vz], type (each
+// n words, FP32 velocities as their bits) into slots [off, off + n) of the
+// SoA buffers, ids 0..n-1, z and vz zero in 2D.
+struct UnpackArgs {
+    uint32_t *px, *py, *pz;
+    float *vx, *vy, *vz;
+    int32_t *type, *id;
+    int off, n, dim;
+};
+
+__global__ __launch_bounds__(kBlock) void k_unpack(UnpackArgs u, const uint32_t *__restrict__ src)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= u.n) return;
+    const size_t n = (size_t)u.n;
+    const bool d3 = u.dim == 3;
+    const uint32_t *v = src + (d3 ? 3 : 2) * n;      // velocities
+    const int s = u.off + i;
+    u.px[s] = src[i];
+    u.py[s] = src[n + i];
+    u.pz[s] = d3 ? src[2 * n + i] : 0u;
+    u.vx[s] = __uint_as_float(v[i]);
+    u.vy[s] = __uint_as_float(v[n + i]);
+    u.vz[s] = d3 ? __uint_as_float(v[2 * n + i]) : 0.0f;
+    u.type[s] = (int32_t)v[(d3 ? 3 : 2) * n + i];
+    u.id[s] = i;
+}
+
 }  // namespace plife
