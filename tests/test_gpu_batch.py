@@ -222,3 +222,36 @@ def test_many_tiny_worlds():
             assert info[k]["pairs"] == e.pair_count()
     finally:
         e.close()
+
+
+def test_full_size_batch_one_step_oracle():
+    """The 512-world batch at full size (~7.9M particles, 3-12 types, rmax and
+    beta per world): after two steps, one more step of every world against
+    the oracle on its own state -- order, exact pair count, velocities and
+    positions within tolerance."""
+    from particle_life import Batch, worlds
+
+    print("\n[batch] case=c4_full_size", flush=True)
+    ws = worlds.config4()
+    b = Batch(0)
+    ids = [b.add_world(w) for w in ws]
+    b.step(2)
+    before = {wid: b.state(wid) for wid in ids}
+    b.step(1)
+    total = 0
+    for wid, w in zip(ids, ws):
+        cur = before[wid]
+        ref = oracle.step(oracle_settings(w), cur["pos"], cur["vel"], cur["types"], ids=cur["ids"],
+                          nthreads=16, debug=True)
+        new = b.state(wid)
+        np.testing.assert_array_equal(new["ids"], ref["ids"], err_msg=f"world {wid}: order")
+        assert b.info(wid)["pairs"] == ref["pair_count"], f"world {wid}: pairs"
+        dv = np.abs(new["vel"] - ref["vel"])
+        assert not (dv > VEL_ABS + VEL_REL * np.abs(ref["vel"])).any(), f"world {wid}: velocity"
+        dp = np.abs(new["pos"] - ref["pos"])
+        dp = np.minimum(dp, 1.0 - dp)
+        assert not (dp > 2.0 ** -32 + w.dt * (VEL_ABS + VEL_REL * np.abs(ref["vel"]))).any(), \
+            f"world {wid}: position"
+        total += ref["pair_count"]
+    print(f"  worlds {len(ws)}, particles {sum(w.n for w in ws)}, pairs {total}", flush=True)
+    b.close()
