@@ -53,7 +53,8 @@ def parse():
                    help="c4/c5: skip the comparison with the plain (one world per handle) path")
     p.add_argument("--n", type=int, default=0, help="particle count override (0 = the config's)")
     p.add_argument("--profile-steps", type=int, default=20)
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-steps", type=int, default=0,
+                   help="steps of the CPU sample; 0 = 100 (c1), 10 (c2), 3 (c3)")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, available cores)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -97,7 +98,7 @@ def cpu_baseline(world, steps: int, threads: int) -> dict:
     t_total = 0.0
     for _ in range(steps):
         t0 = time.perf_counter()
-        r = oracle.step(S, pos, vel, typ, ids=ids, nthreads=threads, debug=True)
+        r = oracle.step(S, pos, vel, typ, ids=ids, nthreads=threads, debug=True, hashes=False)
         t_total += time.perf_counter() - t0
         pairs += r["pair_count"]
         pos, vel, typ, ids = r["pos"], r["vel"], r["types"], r["ids"]
@@ -357,9 +358,13 @@ def main() -> int:
         },
     }
     if not args.no_cpu:
-        cb = cpu_baseline(w, args.cpu_steps, cpu_threads(args.cpu_threads))
+        cpu_steps = args.cpu_steps or {"c1": 100, "c2": 10, "c3": 3}[args.config]
+        cb = cpu_baseline(w, cpu_steps, cpu_threads(args.cpu_threads))
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        # the same restatement on one thread (the reference's LoadDistributor
+        # with a single worker), one step
+        out["cpu_single_thread"] = cpu_baseline(w, 1, 1)
     eng.close()
     print(json.dumps(out))
     return 0

@@ -135,10 +135,12 @@ def ncells(settings: Settings) -> int:
     return nx ** settings.dim
 
 
-def step(settings: Settings, pos, vel, types, ids=None, nthreads: int = 1, debug: bool = False):
+def step(settings: Settings, pos, vel, types, ids=None, nthreads: int = 1, debug: bool = False,
+         hashes: bool = True):
     """One Physics.update().  Returns a dict with the new state in the new
     (cell-sorted) array order and, with debug=True, keys / cell_end / perm /
-    nbr_count / nbr_hash / pair_count."""
+    nbr_count / nbr_hash / pair_count (hashes=False leaves out nbr_hash and
+    its per-pair work, for timing)."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3).copy()
     types = np.ascontiguousarray(types, dtype=np.int32).copy()
@@ -153,13 +155,15 @@ def step(settings: Settings, pos, vel, types, ids=None, nthreads: int = 1, debug
         out["cell_end"] = np.zeros(nc, np.int32)
         out["perm"] = np.zeros(n, np.int32)
         out["nbr_count"] = np.zeros(n, np.int32)
-        out["nbr_hash"] = np.zeros(n, np.uint64)
+        if hashes:
+            out["nbr_hash"] = np.zeros(n, np.uint64)
         dbg = _Debug()
         dbg.keys = out["keys"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         dbg.cell_end = out["cell_end"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         dbg.perm = out["perm"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         dbg.nbr_count = out["nbr_count"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        dbg.nbr_hash = out["nbr_hash"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        if hashes:
+            dbg.nbr_hash = out["nbr_hash"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
     rc = lib().plo_step(ctypes.byref(cs), n, pos.ctypes.data, vel.ctypes.data,
                         types.ctypes.data, ids.ctypes.data, int(nthreads),
                         ctypes.byref(dbg) if dbg is not None else None)

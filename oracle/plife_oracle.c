@@ -229,7 +229,7 @@ static void update_velocity(const step_ctx *c, int i)
                         p->v[1] = p->v[1] + d[1] * scale;
                         p->v[2] = p->v[2] + d[2] * scale;
                         count++;
-                        hash += plo_id_hash((uint32_t)q->id);
+                        if (c->nbr_hash) hash += plo_id_hash((uint32_t)q->id);   /* checker only */
                     }
                 }
             }
