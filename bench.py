@@ -363,8 +363,10 @@ def main() -> int:
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
         # the same restatement on one thread (the reference's LoadDistributor
-        # with a single worker), one step
-        out["cpu_single_thread"] = cpu_baseline(w, 1, 1)
+        # with a single worker), one step; a 4M-particle clustered step would
+        # take minutes on one thread, so C3 has none
+        if args.config in ("c1", "c2"):
+            out["cpu_single_thread"] = cpu_baseline(w, 1, 1)
     eng.close()
     print(json.dumps(out))
     return 0
