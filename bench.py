@@ -55,7 +55,8 @@ def parse():
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-steps", type=int, default=0,
                    help="steps of the CPU sample; 0 = 100 (c1), 10 (c2), 3 (c3)")
-    p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, available cores)")
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="0 = every available CPU (affinity and cgroup quota), at most 16")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="instead of the benchmark, run 5 plain device copies of BYTES bytes "
@@ -75,18 +76,55 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
-def cpu_threads(requested: int) -> int:
+def host_cpus() -> dict:
+    """What the host offers the CPU baseline: logical CPUs, those this process
+    may run on (affinity), the cgroup's CPU quota, and the CPU model."""
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = info["nproc"]
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["model"] = model
+    avail = info["affinity"]
+    if quota:
+        avail = min(avail, max(1, int(quota + 0.5)))
+    info["available"] = avail
+    return info
+
+
+# a GPU box grants one GPU's process this many CPU threads at most
+CPU_SHARE = 16
+
+
+def cpu_threads(requested: int, host: dict) -> int:
     if requested > 0:
         return requested
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
-    return max(1, min(16, avail))
+    return max(1, min(CPU_SHARE, host["available"]))
 
 
-def cpu_baseline(world, steps: int, threads: int) -> dict:
-    """FP64 restatement of Physics.update() on the host, same world."""
+def cpu_baseline(world, steps: int, threads: int, host: dict) -> dict:
+    """FP64 restatement of Physics.update() on the host, same world.  The time
+    is the step proper as plo_step measures it inside (binning and the two
+    parallel phases over `threads` threads), without the wrapper's copies in
+    and out of its particle records."""
     import oracle
 
     S = oracle.Settings(matrix=world.matrix, rmax=world.rmax, friction=world.friction,
@@ -97,9 +135,8 @@ def cpu_baseline(world, steps: int, threads: int) -> dict:
     pairs = 0
     t_total = 0.0
     for _ in range(steps):
-        t0 = time.perf_counter()
         r = oracle.step(S, pos, vel, typ, ids=ids, nthreads=threads, debug=True, hashes=False)
-        t_total += time.perf_counter() - t0
+        t_total += r["seconds"]
         pairs += r["pair_count"]
         pos, vel, typ, ids = r["pos"], r["vel"], r["types"], r["ids"]
     return {
@@ -108,8 +145,10 @@ def cpu_baseline(world, steps: int, threads: int) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": f"{steps} steps of the same N={world.n} world from its initial state, "
-                  f"FP64 C restatement, {threads} threads ({t_total:.2f} s)",
+                  f"FP64 C restatement, {threads} threads, timed inside plo_step "
+                  f"({t_total:.2f} s)",
         "steps_per_s": steps / t_total,
+        "host": host,
     }
 
 
@@ -346,8 +385,9 @@ def main() -> int:
                             "PhysicsSnapshot.write), wall clock"},
         "roofline": {
             "kernel": f"k_force<{w.dim}, wrap, no-debug>",
-            "bound": "mfma",            # the compute roofline (FP32 peak), not HBM
-            "pipe": "valu: FP32 screen + FP64 force law as vector instructions; no matrix ops",
+            "bound": "valu",            # vector ALU issue: sparse gather, no matrix ops
+            "pipe": "FP32 screen + FP64 force law as vector instructions; priced against the "
+                    "FP32 vector peak",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -359,14 +399,22 @@ def main() -> int:
     }
     if not args.no_cpu:
         cpu_steps = args.cpu_steps or {"c1": 100, "c2": 10, "c3": 3}[args.config]
-        cb = cpu_baseline(w, cpu_steps, cpu_threads(args.cpu_threads))
+        host = host_cpus()
+        t_main = cpu_threads(args.cpu_threads, host)
+        cb = cpu_baseline(w, cpu_steps, t_main, host)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        # every CPU the process may use (affinity and cgroup quota), when that
+        # differs from the main sample's count; a box's one-GPU share is
+        # CPU_SHARE threads, so more are not started
+        t_all = min(host["available"], CPU_SHARE)
+        if t_all != t_main and args.config in ("c1", "c2"):
+            out["cpu_all_available"] = cpu_baseline(w, cpu_steps, t_all, host)
         # the same restatement on one thread (the reference's LoadDistributor
         # with a single worker), one step; a 4M-particle clustered step would
         # take minutes on one thread, so C3 has none
         if args.config in ("c1", "c2"):
-            out["cpu_single_thread"] = cpu_baseline(w, 1, 1)
+            out["cpu_single_thread"] = cpu_baseline(w, 1, 1, host)
     eng.close()
     print(json.dumps(out))
     return 0

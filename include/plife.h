@@ -140,14 +140,28 @@ int plife_set_types(plife_physics *h, int32_t preset, uint64_t seed);
 int plife_generate_matrix(int32_t preset, int32_t size, uint64_t seed, double *out);
 
 /* Physics.update() (Physics.java:112-134) applied nsteps times.  Enqueues the
- * steps on the handle's stream and returns; plife_synchronize or a download
- * waits.  Returns PLIFE_STOPPED if plife_force_update_stop() arrived between
- * steps (the state is then at a step boundary). */
+ * steps on the handle's stream; plife_synchronize or a download waits for
+ * them.  To keep a stop prompt, it keeps only about 20 ms of device work
+ * queued (at least 2 steps, at most 62), waiting for the oldest queued
+ * launch before it queues more, and checks the stop flag before every launch
+ * (one launch is 1 or 2 steps).  Returns PLIFE_STOPPED if
+ * plife_force_update_stop() arrived: nothing more is queued, the steps
+ * already queued complete, and the state is then at a step boundary;
+ * plife_get_steps counts them. */
 int plife_update(plife_physics *h, int32_t nsteps);
 
 /* Physics.forceUpdateStop() (Physics.java:149-151).  The only call allowed
- * from another thread; honoured between steps. */
+ * from another thread.  The reference stops before the next particle; here a
+ * step is never cut short, and the stop takes effect before the next launch
+ * (see plife_update).  The flag is cleared when plife_update starts, as
+ * Physics.updateParticles sets its run flag (Physics.java:118). */
 int plife_force_update_stop(plife_physics *h);
+
+/* Steps taken by the handle's world since it was created: every step
+ * plife_update has queued (a queued step always completes).  After
+ * PLIFE_STOPPED and plife_synchronize this says where the world stands.
+ * No counterpart in the reference, whose update() returns only when done. */
+int plife_get_steps(plife_physics *h, int64_t *steps);
 
 int plife_synchronize(plife_physics *h);
 
@@ -246,7 +260,10 @@ int plife_batch_remove_world(plife_batch *b, int32_t world_id);
 int plife_batch_set_world_settings(plife_batch *b, int32_t world_id, const plife_settings *s,
                                    int32_t ntypes, const double *matrix);
 
-/* nsteps steps of every world (enqueued; see plife_update). */
+/* nsteps steps of every world, all enqueued at once (unlike plife_update,
+ * which bounds its queue: a batch's updates are short quanta, and batches on
+ * their own streams overlap only if neither waits).  A stop request is
+ * checked before every launch of this call. */
 int plife_batch_update(plife_batch *b, int32_t nsteps);
 int plife_batch_force_update_stop(plife_batch *b);
 int plife_batch_synchronize(plife_batch *b);

@@ -56,6 +56,7 @@ class _Debug(ctypes.Structure):
         ("nbr_count", ctypes.POINTER(ctypes.c_int32)),
         ("nbr_hash", ctypes.POINTER(ctypes.c_uint64)),
         ("pair_count", ctypes.c_int64),
+        ("seconds", ctypes.c_double),
     ]
 
 
@@ -139,8 +140,10 @@ def step(settings: Settings, pos, vel, types, ids=None, nthreads: int = 1, debug
          hashes: bool = True):
     """One Physics.update().  Returns a dict with the new state in the new
     (cell-sorted) array order and, with debug=True, keys / cell_end / perm /
-    nbr_count / nbr_hash / pair_count (hashes=False leaves out nbr_hash and
-    its per-pair work, for timing)."""
+    nbr_count / nbr_hash / pair_count and seconds, the wall time of the step
+    proper inside plo_step (binning and the two parallel phases, without the
+    copies in and out of its particle records); hashes=False leaves out
+    nbr_hash and its per-pair work, for timing."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3).copy()
     types = np.ascontiguousarray(types, dtype=np.int32).copy()
@@ -172,6 +175,7 @@ def step(settings: Settings, pos, vel, types, ids=None, nthreads: int = 1, debug
     out.update(pos=pos, vel=vel, types=types, ids=ids)
     if debug:
         out["pair_count"] = int(dbg.pair_count)
+        out["seconds"] = float(dbg.seconds)
     return out
 
 

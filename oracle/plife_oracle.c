@@ -29,12 +29,15 @@
  * (Range.java:43,71,86).  It uses a 27-cell stencil ordered z, y, x (outer to
  * inner), key = cx + cy*nx + cz*nx*nx, and wraps/clamps/min-images z like x, y.
  */
+#define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c11 */
+
 #include "plife_oracle.h"
 
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef struct {
     double p[3];
@@ -327,6 +330,9 @@ int plo_step(const plo_settings *s, int n, double *pos, double *vel,
         parts[i].id = id[i];
     }
 
+    struct timespec t_begin, t_end;
+    clock_gettime(CLOCK_MONOTONIC, &t_begin);
+
     /* makeContainers (Physics.java:309-354) */
     for (int i = 0; i < n; i++) {
         keys[i] = cell_key(parts[i].p, cs, nx, dim);
@@ -364,6 +370,7 @@ int plo_step(const plo_settings *s, int n, double *pos, double *vel,
     /* Physics.java:122-131: velocity phase for all, then position phase */
     distribute(&c, n, nthreads, 0);
     distribute(&c, n, nthreads, 1);
+    clock_gettime(CLOCK_MONOTONIC, &t_end);
 
     for (int i = 0; i < n; i++) {
         for (int k = 0; k < 3; k++) {
@@ -380,6 +387,8 @@ int plo_step(const plo_settings *s, int n, double *pos, double *vel,
         int64_t total = 0;
         for (int i = 0; i < n; i++) total += c.nbr_count[i];
         dbg->pair_count = total;
+        dbg->seconds = (double)(t_end.tv_sec - t_begin.tv_sec) +
+                       1e-9 * (double)(t_end.tv_nsec - t_begin.tv_nsec);
     }
     free(tmp_count);
     free(parts);

@@ -39,6 +39,9 @@ typedef struct {
     int32_t *nbr_count;    /* [n]  interacting neighbours, indexed by sorted slot */
     uint64_t *nbr_hash;    /* [n]  sum of plo_id_hash(id) over interacting neighbours */
     int64_t pair_count;    /* total ordered interacting pairs this step */
+    double seconds;        /* wall time of the step proper: makeContainers and the
+                              two parallel phases, without this function's
+                              conversion to and from its particle records */
 } plo_debug;
 
 /* Cells per axis: (int)floor(1/rmax)  (Physics.java:82-85, 312-313). */

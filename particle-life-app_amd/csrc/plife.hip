@@ -117,6 +117,8 @@ struct UploadSlot {
 };
 constexpr int kUploadSlots = 4;
 constexpr int kDlChunks = 8;           // downloads move in at most this many chunks
+constexpr int kFlightSlots = 64;       // events marking queued launches of plife_update
+constexpr double kStopWindowMs = 20.0; // device work plife_update keeps queued (stop latency)
 constexpr int kDlMinChunk = 65536;     // slots; smaller downloads are one chunk
 
 // Grow-only device array: reallocated (contents dropped) when too small.  The
@@ -263,6 +265,12 @@ struct plife_physics {
 
     std::atomic<int> stop{0};
     bool cells_dirty = false;          // cell_count may hold stale counts
+
+    // launches queued by the running plife_update: a ring of events, each
+    // marking the end of one launch of fl_steps[] steps
+    hipEvent_t fl_ev[kFlightSlots] = {};
+    int fl_steps[kFlightSlots] = {};
+    int fl_head = 0, fl_count = 0, fl_queued = 0;
 
     // HIP graph of two consecutive steps (buffer roles repeat every 2 steps)
     hipGraphExec_t gexec = nullptr;
@@ -1293,15 +1301,59 @@ void count_steps(plife_physics *h, int k)
     for (WorldHost &w : h->worlds) w.steps += k;
 }
 
+// Steps plife_update keeps queued on the device.  The reference's stop request
+// takes effect before the next particle (Physics.java:118-133, 149-151); a
+// device step cannot stop halfway, so plife_update queues only about
+// kStopWindowMs of work ahead of the device and checks the stop flag before
+// every launch: a stop then leaves at most that much work to finish.  The
+// step time is estimated from the last measured step (pairs and particles).
+int flight_limit(const plife_physics *h)
+{
+    const double pairs = h->stat_host ? (double)__atomic_load_n(h->stat_host, __ATOMIC_RELAXED) : 0.0;
+    const double est_ms = 0.03 + 4e-9 * pairs + 3e-7 * (double)h->n;
+    const double lim = std::ceil(kStopWindowMs / est_ms);
+    return (int)std::min<double>(std::max(lim, 2.0), (double)kFlightSlots - 2);
+}
+
+// Marks the end of a launch of k steps on the stream; waits for the oldest
+// queued launches while more than `limit` steps are queued.
+int throttle(plife_physics *h, int k, int limit)
+{
+    while (h->fl_count > 0 && (h->fl_queued + k > limit || h->fl_count == kFlightSlots)) {
+        const int s = (h->fl_head - h->fl_count + kFlightSlots) % kFlightSlots;
+        HIP_TRY(hipEventSynchronize(h->fl_ev[s]));
+        h->fl_queued -= h->fl_steps[s];
+        --h->fl_count;
+    }
+    HIP_TRY(hipEventRecord(h->fl_ev[h->fl_head], h->stream));
+    h->fl_steps[h->fl_head] = k;
+    h->fl_head = (h->fl_head + 1) % kFlightSlots;
+    ++h->fl_count;
+    h->fl_queued += k;
+    return PLIFE_OK;
+}
+
 // nsteps steps: pairs of steps replay a captured graph, the rest launch
 // directly.  A graph is captured once the same configuration comes back on a
 // second update (or when one update runs many steps); a configuration that
 // changes every update (worlds joining and leaving) launches directly, as a
-// capture would not be replayed enough to pay for itself.
+// capture would not be replayed enough to pay for itself.  With honour_stop
+// the stop flag is checked before every launch; a single-world handle then
+// also bounds the queued work (flight_limit), so that the steps a stop leaves
+// behind are few.  Batches enqueue every step at once: their updates are
+// short quanta (the serving loop steps at most 32 at a time) and two batches
+// on their own streams must overlap.
 int run_steps(plife_physics *h, int nsteps, bool honour_stop)
 {
     int k = 0, rc;
     if ((rc = prepare_step(h))) return rc;
+    const bool bounded = honour_stop && !h->batch;
+    const int limit = bounded ? flight_limit(h) : 0;
+    h->fl_count = h->fl_queued = 0;
+    auto launched = [&](int steps) -> int {
+        count_steps(h, steps);
+        return bounded ? throttle(h, steps, limit) : PLIFE_OK;
+    };
     bool use_graph = nsteps >= 2 && !h->tune.direct_launch;
     if (use_graph) {
         std::vector<unsigned char> key = graph_key(h);
@@ -1320,13 +1372,13 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
             if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
             HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
             k += 2;
-            count_steps(h, 2);
+            if ((rc = launched(2))) return rc;
         }
     }
     for (; k < nsteps; ++k) {
         if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
         if ((rc = do_step(h, nullptr))) return rc;
-        count_steps(h, 1);
+        if ((rc = launched(1))) return rc;
     }
     return PLIFE_OK;
 }
@@ -1780,6 +1832,10 @@ int create_handle(int32_t device, plife_physics *h)
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
     }
+    for (hipEvent_t &ev : h->fl_ev) {
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+    }
     e = hipHostMalloc((void **)&h->stat_host, 4 * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
     std::memset(h->stat_host, 0, 4 * sizeof(uint64_t));
@@ -1815,6 +1871,10 @@ void destroy_handle(plife_physics *h)
     if (h->pin) (void)hipHostFree(h->pin);
     dev_free(h->dl_dev);
     for (hipEvent_t &ev : h->dl_ev) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
+    for (hipEvent_t &ev : h->fl_ev) {
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
     }
@@ -2062,6 +2122,15 @@ int plife_get_count(plife_physics *h, int32_t *n)
     if (rc) return rc;
     if (!n) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     *n = h->worlds[0].n;
+    return PLIFE_OK;
+}
+
+int plife_get_steps(plife_physics *h, int64_t *steps)
+{
+    int rc = check_single(h);
+    if (rc) return rc;
+    if (!steps) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    *steps = h->worlds[0].steps;
     return PLIFE_OK;
 }
 

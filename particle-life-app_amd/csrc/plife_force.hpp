@@ -412,8 +412,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const int hrow0 = (int)(hkey0 / (uint32_t)nx);
     const int hcx0 = (int)(hkey0 - (uint32_t)hrow0 * (uint32_t)nx);
 
-    // parts = distinct rows among the homes (rows are non-decreasing with tid)
-    const int prow = tid > 0 ? (int)((a.skeys[gs0 - 1] - cb) / (uint32_t)nx) : -1;
+    // parts = distinct rows among the homes (rows are non-decreasing with tid);
+    // only active threads past the first read the previous home's key, so the
+    // read stays inside [g0, g0 + nact)
+    const int prow = (tid > 0 && active0) ? (int)((a.skeys[gs0 - 1] - cb) / (uint32_t)nx) : -1;
     const bool pstart = active0 && (tid == 0 || prow != hrow0);
     const int nrow_next = (tid + 1 < nact) ? (int)((a.skeys[gs0 + 1] - cb) / (uint32_t)nx) : -1;
     const bool pend = active0 && (tid + 1 == nact || nrow_next != hrow0);

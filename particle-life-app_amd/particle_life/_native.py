@@ -34,6 +34,7 @@ EXPORTS = (
     "plife_generate_matrix",
     "plife_update",
     "plife_force_update_stop",
+    "plife_get_steps",
     "plife_synchronize",
     "plife_last_error",
     "plife_get_cells",
@@ -141,6 +142,7 @@ def load() -> ctypes.CDLL:
         "plife_generate_matrix": [i32, i32, ctypes.c_uint64, vp],
         "plife_update": [vp, i32],
         "plife_force_update_stop": [vp],
+        "plife_get_steps": [vp, i64p],
         "plife_synchronize": [vp],
         "plife_get_cells": [vp, vp, vp, vp, i32],
         "plife_get_grid": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)],

@@ -132,6 +132,12 @@ class Engine:
     def stop(self) -> None:
         check(self._lib.plife_force_update_stop(self._h))
 
+    def steps_taken(self) -> int:
+        """Steps queued on this handle since it was created (each completes)."""
+        v = ctypes.c_int64()
+        check(self._lib.plife_get_steps(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def sync(self) -> None:
         check(self._lib.plife_synchronize(self._h))
 

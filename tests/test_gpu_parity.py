@@ -288,28 +288,58 @@ def test_dt_and_matrix_change_between_steps(eng):
     check_one_step(eng, w)
 
 
-@pytest.mark.parametrize("which", ["c2", "c3"])
-def test_full_size_one_step(eng, which):
-    """The benchmark worlds at full size, one step checked against the oracle
-    (keys, ranges, permutation and neighbour sets exactly): the 1M-particle 2D
-    world (~78 interacting neighbours each) and the 4M-particle clustered 3D
-    world of 64 Gaussian blobs (~1,700 each, windowed staging, the clustered
-    launch plan once the first step has measured the clustering)."""
+def test_c2_first_ten_steps_each_checked(eng):
+    """The 1M-particle 2D benchmark world (~78 interacting neighbours each):
+    each of its first ten steps checked against the oracle stepping the same
+    device state -- keys, ranges, permutation and neighbour sets exactly,
+    velocities, accelerations and positions within tolerance."""
     from particle_life import worlds
 
-    print(f"\n[parity] case={which}_full_size", flush=True)
-    if which == "c2":
-        w = worlds.config2()
-        eng.load_world(w)
-        lo, hi = 70, 90
-    else:
-        w = worlds.config3()
-        eng.load_world(w)
-        eng.step(2)                     # the planner sees the clustering after a step
-        lo, hi = 1000, 2500
+    print("\n[parity] case=c2_full_size_ten_steps", flush=True)
+    w = worlds.config2()
+    eng.load_world(w)
+    for k in range(10):
+        st = check_one_step(eng, w, nthreads=16, debug=(k % 3 == 0))
+        print(f"  step {k}: {st}", flush=True)
+        assert 70 * w.n < st["pairs"] < 90 * w.n
+
+
+def test_c3_checked_early_and_late(eng):
+    """The 4M-particle clustered 3D world of 64 Gaussian blobs (~1,700
+    neighbours each, windowed staging, the clustered launch plan once a step
+    has measured the clustering), checked at step 3 and again at step 21,
+    after the blobs have contracted."""
+    from particle_life import worlds
+
+    print("\n[parity] case=c3_full_size", flush=True)
+    w = worlds.config3()
+    eng.load_world(w)
+    eng.step(2)                     # the planner sees the clustering after a step
     st = check_one_step(eng, w, nthreads=16, debug=True)
-    print(f"  {st}", flush=True)
-    assert lo * w.n < st["pairs"] < hi * w.n
+    print(f"  step 3: {st}", flush=True)
+    assert 1000 * w.n < st["pairs"] < 2500 * w.n
+    eng.step(17)
+    st = check_one_step(eng, w, nthreads=16, debug=True)
+    print(f"  step 21: {st}", flush=True)
+    assert 1000 * w.n < st["pairs"] < 4000 * w.n
+
+
+def test_single_particle_world_on_the_staged_kernel(eng):
+    """One particle at slot 0 stepped by the staged force kernel (forced with
+    sparse_worlds=1): a workgroup with a single home at the buffer's start
+    reads no key before it."""
+    print("\n[parity] case=single_particle_staged", flush=True)
+    try:
+        eng.set_tuning(sparse_worlds=1)
+        for n in (1, 2, 3):
+            w = _world(150 + n, n, 3, 0.05, 0.02)
+            eng.load_world(w)
+            for _ in range(2):
+                st = check_one_step(eng, w)
+                if n == 1:
+                    assert st["pairs"] == 0
+    finally:
+        eng.set_tuning()
 
 
 def test_graph_replay_matches_direct_launches(eng):
