@@ -187,6 +187,13 @@ int plife_get_pair_count(plife_physics *h, int64_t *pairs);
 int plife_set_debug(plife_physics *h, int32_t on);
 int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint64_t *hash);
 
+/* Debug mode: the acceleration each particle (sorted slot) received in the
+ * last step -- the sum over its interacting neighbours of the accelerator's
+ * output times rmax * force (Physics.java:437 without the dt), i.e.
+ * (v_new - friction^(60 dt) v_old) / dt before the velocity is stored in
+ * FP32 -- as FP32 (x, y, z) triples, acc[3n]. */
+int plife_get_accelerations(plife_physics *h, int32_t n, float *acc);
+
 /* Launch shape (no counterpart in the reference, which has no device).  Zero
  * fields mean automatic.  block_homes in {0, 64, 128, 256}: home particles
  * (threads) per force workgroup; list_cap in [0, 512]: per-thread LDS

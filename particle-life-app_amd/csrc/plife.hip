@@ -262,6 +262,8 @@ struct plife_physics {
     size_t dbg_cap = 0, dbg_hcap = 0;
     int32_t *dbg_count = nullptr;
     unsigned long long *dbg_hash = nullptr;
+    float *dbg_acc = nullptr;
+    size_t dbg_acap = 0;
 
     std::atomic<int> stop{0};
     bool cells_dirty = false;          // cell_count may hold stale counts
@@ -420,6 +422,7 @@ int ensure_debug(plife_physics *h)
     const size_t need = (size_t)std::max(h->n, 1);
     int rc;
     if ((rc = ensure_dev(&h->dbg_count, &h->dbg_cap, need))) return rc;
+    if ((rc = ensure_dev(&h->dbg_acc, &h->dbg_acap, 3 * need))) return rc;
     return ensure_dev(&h->dbg_hash, &h->dbg_hcap, need);
 }
 
@@ -451,6 +454,7 @@ WorldDev world_dev(const WorldHost &w, uint32_t cell_base, int moff)
     d.inv_1mb = 1.0 / (1.0 - s.beta);
     d.inv_beta = 1.0 / s.beta;
     d.ff = std::pow(s.friction, 60 * s.dt);            // Physics.java:401
+    d.ka = s.force * kInvFix;
     d.kdt = s.force * s.dt / kFix;
     d.dt = s.dt;
     return d;
@@ -1214,6 +1218,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.nb_total = h->nb_total;
     a.dbg_count = h->dbg_count;
     a.dbg_hash = h->dbg_hash;
+    a.dbg_acc = h->dbg_acc;
     h->last_lds = fp.lds;
     h->last_cap = fp.cap;
     h->last_bs = fp.bs;
@@ -1256,7 +1261,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
                        h->vx[1], h->vy[0], h->vy[1], h->vz[0], h->vz[1], h->type[0], h->type[1],
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
-                       h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash,
+                       h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash, h->dbg_acc,
                        h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks,
                        h->stat_host, h->work_host};
     std::memcpy(k.ptr, p, sizeof p);
@@ -1864,6 +1869,7 @@ void destroy_handle(plife_physics *h)
     dev_free(h->d_edits);
     dev_free(h->dbg_count);
     dev_free(h->dbg_hash);
+    dev_free(h->dbg_acc);
     if (h->stat_host) (void)hipHostFree(h->stat_host);
     if (h->work_host) (void)hipHostFree(h->work_host);
     for (void *p : h->retired_host) (void)hipHostFree(p);
@@ -2228,6 +2234,20 @@ int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint
     if (n == 0) return PLIFE_OK;
     if (count) HIP_TRY(hipMemcpy(count, h->dbg_count, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (hash) HIP_TRY(hipMemcpy(hash, h->dbg_hash, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return PLIFE_OK;
+}
+
+int plife_get_accelerations(plife_physics *h, int32_t n, float *acc)
+{
+    if (!h) return fail(PLIFE_ERR_ARGUMENT, "null handle");
+    if (!acc) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (!h->debug || !h->stepped) return fail(PLIFE_ERR_STATE, "debug mode was off or no step has run");
+    if (n != h->n) return fail(PLIFE_ERR_ARGUMENT, "n = %d but the handle holds %d particles", n, h->n);
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (n == 0) return PLIFE_OK;
+    HIP_TRY(hipMemcpy(acc, h->dbg_acc, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost));
     return PLIFE_OK;
 }
 

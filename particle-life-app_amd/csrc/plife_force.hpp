@@ -54,6 +54,7 @@ struct ForceArgs {
     int nb_total;
     int32_t *dbg_count;
     unsigned long long *dbg_hash;
+    float *dbg_acc;                  // debug: the step's acceleration per sorted slot, (x, y, z)
 };
 
 struct Acc {
@@ -234,6 +235,10 @@ __device__ __forceinline__ void finish_home(const ForceArgs &a, const WorldDev &
     if constexpr (DEBUG) {
         a.dbg_count[g] = acc.cnt;
         a.dbg_hash[g] = acc.hash;
+        // the force law's acceleration (Physics.java:437 summed, over dt) in FP32
+        a.dbg_acc[3 * (size_t)g + 0] = (float)(w.ka * acc.x);
+        a.dbg_acc[3 * (size_t)g + 1] = (float)(w.ka * acc.y);
+        a.dbg_acc[3 * (size_t)g + 2] = DIM == 3 ? (float)(w.ka * acc.z) : 0.0f;
     }
 }
 

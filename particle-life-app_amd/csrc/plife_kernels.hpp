@@ -127,6 +127,7 @@ struct WorldDev {
     double beta, one_plus_beta, inv_1mb, inv_beta;
     double ff;                       // friction^(60 dt) (host FP64 pow, Physics.java:401)
     double kdt;                      // force * dt * 2^-32
+    double ka;                       // force * 2^-32: acceleration per unit of accumulated D (f/dist)
     double dt;
 };
 

@@ -42,6 +42,7 @@ EXPORTS = (
     "plife_get_pair_count",
     "plife_set_debug",
     "plife_get_neighbor_summary",
+    "plife_get_accelerations",
     "plife_set_tuning",
     "plife_get_tuning",
     "plife_profile",
@@ -149,6 +150,7 @@ def load() -> ctypes.CDLL:
         "plife_get_pair_count": [vp, i64p],
         "plife_set_debug": [vp, i32],
         "plife_get_neighbor_summary": [vp, i32, vp, vp],
+        "plife_get_accelerations": [vp, i32, vp],
         "plife_set_tuning": [vp, ctypes.POINTER(PlifeTuning)],
         "plife_get_tuning": [vp, ctypes.POINTER(PlifeTuning), ctypes.POINTER(i32)],
         "plife_profile": [vp, i32, ctypes.POINTER(PlifeTiming)],

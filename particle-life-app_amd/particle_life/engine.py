@@ -170,6 +170,12 @@ class Engine:
         check(self._lib.plife_get_neighbor_summary(self._h, self.n, count.ctypes.data, h.ctypes.data))
         return count, h
 
+    def accelerations(self) -> np.ndarray:
+        """Debug mode: each particle's acceleration of the last step, FP32 (n, 3)."""
+        acc = np.zeros((self.n, 3), np.float32)
+        check(self._lib.plife_get_accelerations(self._h, self.n, acc.ctypes.data))
+        return acc
+
     # launch tuning (results do not depend on it)
     def set_tuning(self, list_cap: int = 0, stage_slack: float = 0.0,
                    direct_launch: bool = False, block_homes: int = 0,
