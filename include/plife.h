@@ -220,6 +220,35 @@ int plife_set_tuning(plife_physics *h, const plife_tuning *t);
  * workgroup of the last force launch. */
 int plife_get_tuning(plife_physics *h, plife_tuning *t, int32_t *lds_bytes);
 
+/* One Physics.update() (Physics.java:112-134) of one world whose state lives
+ * in caller-owned device memory, enqueued on `stream` (a hipStream_t; NULL =
+ * the default stream), that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It runs
+ * plife_update's kernels, with every buffer they need beyond the state in a
+ * static device array of the library (19.1 MB, held by each GPU the
+ * library runs on until the process ends); it calls no hipMalloc, hipFree
+ * or memset/memcpy, only kernel launches.
+ *   - State in the device layout (plife_update's): positions as 32-bit
+ *     fixed point (x = k * 2^-32 of the unit box), velocities FP32, types and
+ *     ids int32, SoA arrays of n; pz, vz, opz, ovz only in 3D (else NULL).
+ *     Inputs in any order; outputs in the new cell-sorted order (the
+ *     reference's particle array after makeContainers' swap), ids carried.
+ *     Inputs and outputs must not overlap.  Positions must already lie in
+ *     the box (as plife_set_particles leaves them).
+ *   - matrix: ntypes * ntypes FP64 in device memory, row = acted-on type.
+ *   - pairs: optional device uint64 receiving the interacting pair count.
+ *   - Limits: n <= 262144, nx^dim <= 2^20 cells (nx = floor(1/rmax));
+ *     larger worlds return PLIFE_ERR_ARGUMENT.
+ *   - Calls that overlap on one GPU share its array and race: serialise
+ *     them (one stream, or events).  Each call clears what must start at zero.
+ * Same results as plife_update stepping the same state, bit for bit. */
+int plife_step_noalloc(const plife_settings *s, int32_t ntypes, const double *matrix, int32_t n,
+                       const uint32_t *px, const uint32_t *py, const uint32_t *pz, const float *vx,
+                       const float *vy, const float *vz, const int32_t *type, const int32_t *id,
+                       uint32_t *opx, uint32_t *opy, uint32_t *opz, float *ovx, float *ovy,
+                       float *ovz, int32_t *otype, int32_t *oid, uint64_t *pairs, void *stream);
+
 /* Runs nsteps steps with HIP events around each phase and fills *out. */
 int plife_profile(plife_physics *h, int32_t nsteps, plife_timing *out);
 

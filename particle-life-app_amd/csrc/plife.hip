@@ -23,6 +23,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <sched.h>
 #include <string>
 #include <thread>
@@ -903,25 +904,10 @@ int record(plife_physics *h, PhaseEvents *ev, int k)
 //     a CU while the dense neighbourhoods stream through it in windows.
 // The plan never changes results: every home visits its candidates in the
 // same order whatever the windows and workgroup size.
-ForcePlan plan_force(const plife_physics *h)
+ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const plife_tuning &t,
+                         bool dbg, int ntypes_max)
 {
-    const int dim = h->dim == 3 ? 3 : 2;
     const int NR = dim == 3 ? 9 : 3;
-    // density of the worlds the staged kernel steps (all of them if none is)
-    long long sn = 0, sc = 0;
-    for (const WorldHost &w : h->worlds)
-        if (!w.sparse) {
-            sn += w.n;
-            sc += grid_of(w.st).ncells;
-        }
-    if (sc == 0) {
-        sn = h->n;
-        sc = h->ncells;
-    }
-    const double avg = (double)sn / (double)std::max(sc, 1LL);
-    const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
-    const bool clustered = h->clustered;
-    const plife_tuning &t = h->tune;
     ForcePlan p;
     p.bs = t.block_homes ? t.block_homes : (clustered ? 128 : 256);
     p.lcap = t.list_cap > 0 ? std::max(8, std::min(t.list_cap, 512))   // >= the screen group (8)
@@ -943,9 +929,9 @@ ForcePlan plan_force(const plife_physics *h)
         budget = kClusteredLds;
     }
     want = std::min<long long>((want + 63) / 64 * 64, 65535);
-    const bool dbg = h->debug != 0, m_in_lds = h->ntypes_max <= kMaxTypesLds;
+    const bool m_in_lds = ntypes_max <= kMaxTypesLds;
     auto total = [&](long long c) {
-        return force_lds_layout(dim, dbg, h->ntypes_max, m_in_lds, (int)c, p.lcap, p.bs,
+        return force_lds_layout(dim, dbg, ntypes_max, m_in_lds, (int)c, p.lcap, p.bs,
                                 p.max_cells).total;
     };
     // keep room for a useful staging window: shrink the cell table first
@@ -956,6 +942,25 @@ ForcePlan plan_force(const plife_physics *h)
     p.cap = (int)std::max(0LL, want);
     p.lds = total(p.cap);
     return p;
+}
+
+ForcePlan plan_force(const plife_physics *h)
+{
+    const int dim = h->dim == 3 ? 3 : 2;
+    // density of the worlds the staged kernel steps (all of them if none is)
+    long long sn = 0, sc = 0;
+    for (const WorldHost &w : h->worlds)
+        if (!w.sparse) {
+            sn += w.n;
+            sc += grid_of(w.st).ncells;
+        }
+    if (sc == 0) {
+        sn = h->n;
+        sc = h->ncells;
+    }
+    const double avg = (double)sn / (double)std::max(sc, 1LL);
+    const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
+    return plan_force_for(dim, avg, occ, h->clustered, h->tune, h->debug != 0, h->ntypes_max);
 }
 
 // Read the clustering measure of the latest step the host has seen complete
@@ -1939,6 +1944,256 @@ void report_tuning(const plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
     if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
 }
 
+// ---------------------------------------------------------------------------
+// the step without allocation (plife_step_noalloc)
+// ---------------------------------------------------------------------------
+//
+// Everything a step needs beyond the caller's state arrays lives in one
+// file-scope device array, `scratch`, of fixed size: the world's constants,
+// the workgroup table, the step's counters, the keys, radix buffers and
+// permutation, and the cell arrays.  Each GPU the library runs on holds its
+// own copy of the array until the process ends.  Host code never passes the
+// array's name to a kernel: the address each device holds it at comes from
+// hipGetSymbolAddress (na_scratch_base) and only that address is used.
+
+constexpr int kNaMaxN = 262144;                  // particles per call
+constexpr long long kNaMaxCells = 1LL << 20;     // cells per call (nx^dim)
+constexpr int kNaMaxBlocks = kNaMaxN / 64;       // force workgroups (64 homes or more each)
+
+struct NaLayout {
+    size_t world, counters, blocks, work, keys0, kA, kB, vA, vB, skeys, count, start, end, hist,
+        bsum, bsq, total;
+};
+
+constexpr size_t na_align(size_t o) { return (o + 255) & ~(size_t)255; }
+
+constexpr NaLayout na_layout()
+{
+    NaLayout L{};
+    size_t o = 0;
+    L.world = o;    o = na_align(o + sizeof(WorldDev));
+    L.counters = o; o = na_align(o + 8 * sizeof(unsigned long long));
+    L.blocks = o;   o = na_align(o + (size_t)kNaMaxBlocks * sizeof(uint2));
+    L.work = o;     o = na_align(o + ((size_t)kNaMaxBlocks + 1) * sizeof(uint32_t));
+    L.keys0 = o;    o = na_align(o + (size_t)kNaMaxN * 4);
+    L.kA = o;       o = na_align(o + (size_t)kNaMaxN * 4);
+    L.kB = o;       o = na_align(o + (size_t)kNaMaxN * 4);
+    L.vA = o;       o = na_align(o + (size_t)kNaMaxN * 4);
+    L.vB = o;       o = na_align(o + (size_t)kNaMaxN * 4);
+    L.skeys = o;    o = na_align(o + (size_t)kNaMaxN * 4);
+    L.count = o;    o = na_align(o + (size_t)kNaMaxCells * 4);
+    L.start = o;    o = na_align(o + (size_t)kNaMaxCells * 4);
+    L.end = o;      o = na_align(o + (size_t)kNaMaxCells * 4);
+    // radix digit histograms: at most 2^8 digits per tile (passes of <= 8 bits)
+    L.hist = o;     o = na_align(o + (size_t)256 * ((kNaMaxN + kRadixTile - 1) / kRadixTile) * 4);
+    L.bsum = o;     o = na_align(o + (size_t)4096 * 4);
+    L.bsq = o;      o = na_align(o + (size_t)((kNaMaxCells + kScanTile - 1) / kScanTile) * 8);
+    L.total = o;
+    return L;
+}
+
+constexpr NaLayout kNa = na_layout();
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNa.total];
+
+namespace {
+
+std::mutex g_na_mu;
+unsigned char *g_na_base[64] = {};
+
+// The address device `dev` holds `scratch` at, looked up once per device
+// with that device current.
+int na_scratch_base(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= 64) return fail(PLIFE_ERR_ARGUMENT, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_base[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PLIFE_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_na_base[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_na_base[dev];
+    return PLIFE_OK;
+}
+
+// Per call: the world's constants, the workgroup table (world 0, first home
+// b * bs), and zeros where the step expects them -- the counters, the tag
+// slot of the workgroup work record, and the first ncells cell counts (a
+// previous call may have stopped between binning and the cell scan).
+__global__ void k_na_setup(WorldDev wd, WorldDev *world, uint2 *blocks, int nblocks, int bs,
+                           unsigned long long *counters, uint32_t *work, uint32_t *cell_count,
+                           int ncells)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *world = wd;
+    if (i < 8) counters[i] = 0ull;
+    if (i < nblocks) blocks[i] = make_uint2(0u, (uint32_t)(i * bs));
+    if (i == 0) work[nblocks] = 0u;
+    for (int c = i; c < ncells; c += gridDim.x * blockDim.x) cell_count[c] = 0u;
+}
+
+__global__ void k_na_copy_pairs(const unsigned long long *counters, unsigned long long *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = counters[0];
+}
+
+int noalloc_step(const plife_settings *s, int32_t ntypes, const double *matrix, int32_t n,
+                 const uint32_t *px, const uint32_t *py, const uint32_t *pz, const float *vx,
+                 const float *vy, const float *vz, const int32_t *type, const int32_t *id,
+                 uint32_t *opx, uint32_t *opy, uint32_t *opz, float *ovx, float *ovy, float *ovz,
+                 int32_t *otype, int32_t *oid, uint64_t *pairs, hipStream_t stream)
+{
+    if (!s || !matrix) return fail(PLIFE_ERR_ARGUMENT, "null argument");
+    if (s->dim != 2 && s->dim != 3) return fail(PLIFE_ERR_ARGUMENT, "dim must be 2 or 3, got %d", s->dim);
+    if (!(s->rmax > 0.0) || !(s->rmax <= 1.0))
+        return fail(PLIFE_ERR_ARGUMENT, "rmax must be in (0, 1], got %g", s->rmax);
+    if (!(s->beta > 0.0) || !(s->beta < 1.0))
+        return fail(PLIFE_ERR_ARGUMENT, "beta must be in (0, 1), got %g", s->beta);
+    if (!std::isfinite(s->dt) || !std::isfinite(s->force) || !std::isfinite(s->friction))
+        return fail(PLIFE_ERR_ARGUMENT, "dt, force and friction must be finite");
+    if (ntypes < 1 || ntypes > kMaxTypes)
+        return fail(PLIFE_ERR_ARGUMENT, "matrix size must be in [1, %d], got %d", kMaxTypes, ntypes);
+    if (n < 0 || n > kNaMaxN)
+        return fail(PLIFE_ERR_ARGUMENT, "n = %d exceeds the scratch limit of %d particles", n, kNaMaxN);
+    const Grid gr = grid_of(*s);
+    if (gr.ncells > kNaMaxCells)
+        return fail(PLIFE_ERR_ARGUMENT, "%lld cells (rmax %g, dim %d) exceed the scratch limit of %lld",
+                    gr.ncells, s->rmax, s->dim, kNaMaxCells);
+    const bool d3 = s->dim == 3;
+    if (n > 0 && (!px || !py || !vx || !vy || !type || !id || !opx || !opy || !ovx || !ovy ||
+                  !otype || !oid || (d3 && (!pz || !vz || !opz || !ovz))))
+        return fail(PLIFE_ERR_ARGUMENT, "null state array");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    unsigned char *base = nullptr;
+    int rc = na_scratch_base(dev, &base);
+    if (rc) return rc;
+
+    WorldHost wh;
+    wh.st = *s;
+    wh.ntypes = ntypes;
+    wh.n = n;
+    wh.off = 0;
+    const WorldDev wd = world_dev(wh, 0u, 0);
+    const int nc = (int)gr.ncells;
+    const double avg = (double)n / (double)nc;
+    const ForcePlan fp = plan_force_for(s->dim, avg, avg + 1.0, false, plife_tuning{}, false, ntypes);
+    const int nblocks = ceil_div(n, fp.bs);
+    const RadixPlan plan = radix_plan(nc, n);
+
+    // a transient view of the scratch as the step helpers' buffers (no
+    // allocation: plain pointers into the array)
+    plife_physics t;
+    t.stream = stream;
+    t.dim = s->dim;
+    t.n = n;
+    t.ncells = nc;
+    t.ntypes_max = ntypes;
+    t.d_worlds = reinterpret_cast<WorldDev *>(base + kNa.world);
+    t.d_pairs = reinterpret_cast<unsigned long long *>(base + kNa.counters);
+    t.d_blocks = reinterpret_cast<uint2 *>(base + kNa.blocks);
+    t.keys0 = reinterpret_cast<uint32_t *>(base + kNa.keys0);
+    t.kA = reinterpret_cast<uint32_t *>(base + kNa.kA);
+    t.kB = reinterpret_cast<uint32_t *>(base + kNa.kB);
+    t.vA = reinterpret_cast<uint32_t *>(base + kNa.vA);
+    t.vB = reinterpret_cast<uint32_t *>(base + kNa.vB);
+    t.skeys = reinterpret_cast<uint32_t *>(base + kNa.skeys);
+    t.cell_count = reinterpret_cast<uint32_t *>(base + kNa.count);
+    t.cell_start = reinterpret_cast<uint32_t *>(base + kNa.start);
+    t.cell_end = reinterpret_cast<uint32_t *>(base + kNa.end);
+    t.hist = reinterpret_cast<uint32_t *>(base + kNa.hist);
+    t.bsum = reinterpret_cast<uint32_t *>(base + kNa.bsum);
+    t.bsum_cap = (kNa.bsq - kNa.bsum) / 4;
+    t.bsq = reinterpret_cast<unsigned long long *>(base + kNa.bsq);
+    t.bsq_cap = (kNa.total - kNa.bsq) / 8;
+    uint32_t *work = reinterpret_cast<uint32_t *>(base + kNa.work);
+    if ((size_t)ceil_div(nc, kScanTile) + (size_t)ceil_div((long long)plan.hist_len, kScanTile) >
+            t.bsum_cap ||
+        plan.hist_len > (kNa.bsum - kNa.hist) / 4 || nblocks > kNaMaxBlocks)
+        return fail(PLIFE_ERR_STATE, "scratch layout too small for this world");
+
+    const int setup_threads = std::max(nblocks + 1, 8);
+    k_na_setup<<<std::min(ceil_div(std::max(setup_threads, nc), kBlock), 1024), kBlock, 0, stream>>>(
+        wd, t.d_worlds, t.d_blocks, nblocks, fp.bs, t.d_pairs, work, t.cell_count, nc);
+    LAUNCH_CHECK();
+    if (n > 0) {
+        // 1. cell keys, histogram and first digit histogram; counters [0, 3)
+        //    cleared, tag in [3] (Physics.java:319-340)
+#define PLIFE_NA_BIN(DD)                                                                          \
+    do {                                                                                          \
+        if (d3) k_bin_tile<3, DD><<<plan.ntiles, 1024, 0, stream>>>(px, py, pz, n, t.d_worlds,   \
+                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull);       \
+        else k_bin_tile<2, DD><<<plan.ntiles, 1024, 0, stream>>>(px, py, pz, n, t.d_worlds,      \
+                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull);       \
+    } while (0)
+        switch (plan.D) {
+            case 1: PLIFE_NA_BIN(1); break;
+            case 2: PLIFE_NA_BIN(2); break;
+            case 3: PLIFE_NA_BIN(3); break;
+            case 4: PLIFE_NA_BIN(4); break;
+            case 5: PLIFE_NA_BIN(5); break;
+            case 6: PLIFE_NA_BIN(6); break;
+            case 7: PLIFE_NA_BIN(7); break;
+            default: PLIFE_NA_BIN(8); break;
+        }
+#undef PLIFE_NA_BIN
+        LAUNCH_CHECK();
+        // cell scan (clears the counts) and first digit scan
+        if ((rc = scan_jobs(&t, ScanJob{t.cell_count, nc, t.cell_start, t.cell_end, 1, nullptr,
+                                        t.d_pairs + 1},
+                            ScanJob{t.hist, (int)plan.hist_len, t.hist, nullptr, 0, nullptr, nullptr})))
+            return rc;
+        // 2. the stable sort: the reference's counting-sort permutation
+        if ((rc = radix_sort(&t, n, plan))) return rc;
+        // 3. force + integrate into the caller's output arrays
+        ForceArgs a{};
+        a.perm = t.perm;
+        a.sx = px;
+        a.sy = py;
+        a.sz = pz;
+        a.vx = vx;
+        a.vy = vy;
+        a.vz = vz;
+        a.type = type;
+        a.id = id;
+        a.ox = opx;
+        a.oy = opy;
+        a.oz = opz;
+        a.ovx = ovx;
+        a.ovy = ovy;
+        a.ovz = ovz;
+        a.otype = otype;
+        a.oid = oid;
+        a.skeys = t.skeys;
+        a.cell_start = t.cell_start;
+        a.cell_end = t.cell_end;
+        a.matrix = matrix;
+        a.worlds = t.d_worlds;
+        a.blocks = t.d_blocks;
+        a.ntypes_max = ntypes;
+        a.m_in_lds = ntypes <= kMaxTypesLds ? 1 : 0;
+        a.cap = fp.cap;
+        a.lcap = fp.lcap;
+        a.max_cells = fp.max_cells;
+        a.pair_total = t.d_pairs;
+        a.pair_world = t.d_pairs + 2;
+        a.block_work = work;
+        a.work_tag = 1u;
+        a.nb_total = nblocks;
+        t.seg[s->wrap ? 0 : 2] = nblocks;
+        if ((rc = launch_force_any(&t, a, fp.bs, fp.lds))) return rc;
+    }
+    if (pairs) {
+        k_na_copy_pairs<<<1, 64, 0, stream>>>(t.d_pairs, reinterpret_cast<unsigned long long *>(pairs));
+        LAUNCH_CHECK();
+    }
+    return PLIFE_OK;
+}
+
 WorldHost *find_world(plife_physics *h, int32_t world_id)
 {
     auto it = h->index_of.find(world_id);
@@ -2235,6 +2490,16 @@ int plife_get_neighbor_summary(plife_physics *h, int32_t n, int32_t *count, uint
     if (count) HIP_TRY(hipMemcpy(count, h->dbg_count, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (hash) HIP_TRY(hipMemcpy(hash, h->dbg_hash, (size_t)n * 8, hipMemcpyDeviceToHost));
     return PLIFE_OK;
+}
+
+int plife_step_noalloc(const plife_settings *s, int32_t ntypes, const double *matrix, int32_t n,
+                       const uint32_t *px, const uint32_t *py, const uint32_t *pz, const float *vx,
+                       const float *vy, const float *vz, const int32_t *type, const int32_t *id,
+                       uint32_t *opx, uint32_t *opy, uint32_t *opz, float *ovx, float *ovy,
+                       float *ovz, int32_t *otype, int32_t *oid, uint64_t *pairs, void *stream)
+{
+    return noalloc_step(s, ntypes, matrix, n, px, py, pz, vx, vy, vz, type, id, opx, opy, opz, ovx,
+                        ovy, ovz, otype, oid, pairs, static_cast<hipStream_t>(stream));
 }
 
 int plife_get_accelerations(plife_physics *h, int32_t n, float *acc)

@@ -43,6 +43,7 @@ EXPORTS = (
     "plife_set_debug",
     "plife_get_neighbor_summary",
     "plife_get_accelerations",
+    "plife_step_noalloc",
     "plife_set_tuning",
     "plife_get_tuning",
     "plife_profile",
@@ -151,6 +152,7 @@ def load() -> ctypes.CDLL:
         "plife_set_debug": [vp, i32],
         "plife_get_neighbor_summary": [vp, i32, vp, vp],
         "plife_get_accelerations": [vp, i32, vp],
+        "plife_step_noalloc": [ctypes.POINTER(PlifeSettings), i32, vp, i32] + [vp] * 16 + [vp, vp],
         "plife_set_tuning": [vp, ctypes.POINTER(PlifeTuning)],
         "plife_get_tuning": [vp, ctypes.POINTER(PlifeTuning), ctypes.POINTER(i32)],
         "plife_profile": [vp, i32, ctypes.POINTER(PlifeTiming)],
