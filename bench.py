@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="0 = every available CPU (affinity and cgroup quota), at most 16")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--tuning", default="",
+                   help="c1-c3: launch tuning as JSON, e.g. '{\"windowed_only\": true}' "
+                        "(fields of plife_tuning; results do not depend on it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="instead of the benchmark, run 5 plain device copies of BYTES bytes "
                         "(a known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -301,6 +304,8 @@ def main() -> int:
     make = {"c1": worlds.config1, "c2": worlds.config2, "c3": worlds.config3}[args.config]
     w = make(args.n) if args.n else make()
     eng = Engine(0)
+    if args.tuning:
+        eng.set_tuning(**json.loads(args.tuning))
     eng.load_world(w)
     eng.step(args.warmup)
     eng.sync()

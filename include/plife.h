@@ -203,7 +203,10 @@ int plife_get_accelerations(plife_physics *h, int32_t n, float *acc);
  * graph of two steps; sparse_worlds: which worlds the sparse-world force
  * kernel (one thread per particle, stencil read from global memory) steps
  * instead of the staged one -- 0 those whose cells hold too few particles
- * for staging to pay, 1 none, 2 all.  Results do not depend on the tuning.  Automatic
+ * for staging to pay, 1 none, 2 all; windowed_only 1: the staged force
+ * kernel always screens homes in windows with per-thread candidate lists,
+ * never on its balanced path (one window, homes dealt to lanes by their
+ * kept-candidate counts).  Results do not depend on the tuning.  Automatic
  * fields follow the world's clustering: every step measures the mean size of
  * the cell a particle sits in, and each plife_update plans its launches from
  * the latest measurement that has reached the host. */
@@ -213,6 +216,7 @@ typedef struct {
     int32_t direct_launch;
     double stage_slack;
     int32_t sparse_worlds;
+    int32_t windowed_only;
 } plife_tuning;
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t);

@@ -70,8 +70,10 @@ constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
+constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
+    int wcap = 0;                     // mask words per home of the balanced path (0: windowed only)
     int bs, lcap, max_cells, cap;
     size_t lds;
 };
@@ -917,6 +919,15 @@ ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const 
     // rows of (span + 2) cells.  Clustered worlds keep a short cell table: the
     // few sparse workgroups that span more cells take the global path.
     const double span = p.bs / std::max(avg, 1e-9) + 1.0;
+    // balanced path (uniform worlds): a home's stencil row is a run of three
+    // cells, ~3 avg candidates; its mask words must hold the longest run of a
+    // workgroup (workgroups where one does not fit take the windowed path)
+    if (!clustered && !t.windowed_only) {
+        const double run = 3.0 * avg;
+        const double run_hi = run + 4.0 * std::sqrt(run) + 3.0;
+        const int wcap = NR * (int)std::ceil(run_hi / 32.0);
+        if ((size_t)wcap * p.bs * 6 <= kWordLds) p.wcap = wcap;
+    }
     p.max_cells = (int)std::min<double>(std::max(64.0, std::ceil(NR * (span + 2.0) * 1.5 + 16)),
                                         clustered ? 384.0 : 2048.0);
     long long want;
@@ -932,7 +943,7 @@ ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const 
     const bool m_in_lds = ntypes_max <= kMaxTypesLds;
     auto total = [&](long long c) {
         return force_lds_layout(dim, dbg, ntypes_max, m_in_lds, (int)c, p.lcap, p.bs,
-                                p.max_cells).total;
+                                p.max_cells, p.wcap).total;
     };
     // keep room for a useful staging window: shrink the cell table first
     const long long min_cap = std::min<long long>(want, 256);
@@ -1216,6 +1227,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.cap = fp.cap;
     a.lcap = fp.lcap;
     a.max_cells = fp.max_cells;
+    a.wcap = fp.wcap;
     a.pair_total = h->d_pairs;
     a.pair_world = h->d_pairs + 2;
     a.block_work = h->work_dev;
@@ -1928,7 +1940,8 @@ int check_tuning(const plife_tuning *t)
     if (!t) return fail(PLIFE_ERR_ARGUMENT, "null argument");
     if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
         t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
-        t->sparse_worlds < 0 || t->sparse_worlds > 2 ||
+        t->sparse_worlds < 0 || t->sparse_worlds > 2 || t->windowed_only < 0 ||
+        t->windowed_only > 1 ||
         (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
          t->block_homes != 256))
         return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
@@ -2179,6 +2192,7 @@ int noalloc_step(const plife_settings *s, int32_t ntypes, const double *matrix, 
         a.cap = fp.cap;
         a.lcap = fp.lcap;
         a.max_cells = fp.max_cells;
+        a.wcap = fp.wcap;
         a.pair_total = t.d_pairs;
         a.pair_world = t.d_pairs + 2;
         a.block_work = work;

@@ -47,6 +47,7 @@ struct ForceArgs {
     int block_base;                  // this launch's first entry of blocks
     int ntypes_max, m_in_lds;
     int cap, lcap, max_cells;
+    int wcap;                        // mask words per home of the balanced path (0: windowed only)
     unsigned long long *pair_total;  // interacting pairs of the step, all worlds
     unsigned long long *pair_world;  // per world
     uint32_t *block_work;            // pinned host memory: per table entry its pairs (saturated); [nb_total] = tag
@@ -343,11 +344,15 @@ __device__ __forceinline__ int deal_key(uint32_t y, uint32_t z, int hrow, int nx
 }
 
 struct ForceLds {
-    size_t m, sxy, sq, lists, sz, sqz, sid, off, gs, st, total;
+    size_t m, sxy, sq, lists, wbase, sz, sqz, sid, off, gs, st, total;
 };
 
+// The candidate lists (windowed path: lcap u16 entries per thread) and the
+// mask words (balanced path: wcap u32 masks + wcap u16 first indices per
+// thread) share one region; a workgroup takes one path.
 __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int ntypes, bool m_in_lds,
-                                                     int cap, int lcap, int bs, int max_cells)
+                                                     int cap, int lcap, int bs, int max_cells,
+                                                     int wcap)
 {
     ForceLds L;
     size_t o = 0;
@@ -356,7 +361,12 @@ __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int nt
     // a group of U candidates from one base, up to U - 1 past a lane's range
     L.sxy = o;   o += (size_t)(cap + kScreenPad) * 8;
     L.sq = o;    o += dim == 3 ? (size_t)(cap + kScreenPad) * 4 : 0;     // 16-bit screen (3D only)
-    L.lists = o; o += (size_t)lcap * bs * 2;
+    L.lists = o;
+    L.wbase = o + (size_t)wcap * bs * 4;
+    {
+        const size_t lsz = (size_t)lcap * bs * 2, wsz = (size_t)wcap * bs * 6;
+        o += lsz > wsz ? lsz : wsz;
+    }
     o = (o + 3) & ~(size_t)3;
     L.sz = o;    o += dim == 3 ? (size_t)(cap + kScreenPad) * 4 : 0;
     L.sqz = o;   o += dim == 3 ? (size_t)(cap + kScreenPad) * 2 : 0;
@@ -376,21 +386,29 @@ template <int DIM, bool WRAP, bool DEBUG, bool MLDS, int BS>
 __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
 {
     constexpr int NR = DIM == 3 ? 9 : 3;      // stencil rows per part
+    constexpr int RMID = DIM == 3 ? 4 : 1;    // the stencil row of the home's own row (oy = oz = 0)
     constexpr int U = 8;                      // screen group; lists keep U slots free (lcap >= U)
     constexpr int NW = BS / kWave;
+    constexpr int PER = 256 / BS;             // count bins per thread in the balanced deal
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_part[kMaxParts][3];      // row, first column, last column
     __shared__ uint32_t s_cnt[NW + 1];
     __shared__ unsigned long long s_pairs[NW];
     __shared__ uint32_t s_qc[NW][kDealKeys];
     __shared__ uint32_t s_deal[BS];
+    // balanced path: per home its staged index, kept candidates and mask words
+    __shared__ uint16_t s_si[BS], s_hc[BS];
+    __shared__ uint32_t s_bin[256];
 
-    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells);
+    const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells,
+                                        a.wcap);
     double *M = reinterpret_cast<double *>(smem + L.m);
     uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
     uint32_t *sq = reinterpret_cast<uint32_t *>(smem + L.sq);
     uint16_t *sqz = reinterpret_cast<uint16_t *>(smem + L.sqz);
     uint16_t *lists = reinterpret_cast<uint16_t *>(smem + L.lists);
+    uint32_t *wmask = reinterpret_cast<uint32_t *>(smem + L.lists);     // balanced path: same region
+    uint16_t *wbase = reinterpret_cast<uint16_t *>(smem + L.wbase);
     uint32_t *szz = reinterpret_cast<uint32_t *>(smem + L.sz);
     uint32_t *sid = reinterpret_cast<uint32_t *>(smem + L.sid);
     uint32_t *off = reinterpret_cast<uint32_t *>(smem + L.off);
@@ -440,44 +458,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         s_part[mypart0][1] = hcx0;
     }
     if (pend && mypart0 < kMaxParts) s_part[mypart0][2] = hcx0;
-
-    // Deal the homes to threads grouped by where they sit in their cell (y
-    // quartile in 2D, y/z halves in 3D; stable within a group).  A home's
-    // accepted candidates bunch in the stencil rows nearest to it, so lanes
-    // of one wave then fill their candidate lists at similar rates and list
-    // flushes keep more lanes busy.  Each home's own work is unchanged.
-    const int dkey = active0 ? deal_key<DIM>(a.sy[a.perm[gs0]], DIM == 3 ? a.sz[a.perm[gs0]] : 0u,
-                                             hrow0, nx, w.cs)
-                             : kDealKeys - 1;
-    const uint64_t lt = (1ull << lane) - 1;
-    uint32_t myrank = 0;
-#pragma unroll
-    for (int q = 0; q < kDealKeys; ++q) {
-        const uint64_t b = __ballot(dkey == q);
-        if (lane == 0) s_qc[wv][q] = (uint32_t)__popcll(b);
-        if (dkey == q) myrank = (uint32_t)__popcll(b & lt);
-    }
     __syncthreads();
-    for (int q = 0; q < kDealKeys; ++q)
-#pragma unroll
-        for (int v = 0; v < NW; ++v)
-            myrank += (q < dkey || (q == dkey && v < wv)) ? s_qc[v][q] : 0u;
-    s_deal[myrank] = (uint32_t)tid | ((uint32_t)max(0, min(mypart0, 255)) << 16);
-    __syncthreads();
-
-    // this thread's home after the deal
-    const uint32_t dealt = s_deal[tid];
-    const int th = (int)(dealt & 0xFFFFu);
-    const int mypart = (int)(dealt >> 16);
-    const bool active = th < nact;
-    const int g = g0 + (active ? th : nact - 1);
-    const uint32_t hkey = a.skeys[g] - cb;
-    const int hrow = (int)(hkey / (uint32_t)nx);
-    const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
-    const uint32_t ho = a.perm[g];
-    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
-    const int htype = a.type[ho];
-    const int mrow = htype * w.ntypes;
 
     int p_row[kMaxParts], p_ca[kMaxParts], p_w[kMaxParts], p_base[kMaxParts + 1];
     p_base[0] = 0;
@@ -490,7 +471,119 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         p_base[p + 1] = p_base[p] + NR * p_w[p];
     }
     const int NC = p_base[kMaxParts];
-    bool use_lds = !w.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
+    const bool use_lds = !w.all_global && a.cap > 0 && nparts <= (uint32_t)kMaxParts && NC <= a.max_cells;
+
+    // staged cells: start and count, then an exclusive prefix in LDS (carry ends
+    // as the staged particle count, the same in every thread)
+    uint32_t carry = 0;
+    if (use_lds) {
+        for (int f0 = 0; f0 < NC; f0 += BS) {
+            const int f = f0 + tid;
+            uint32_t c = 0;
+            if (f < NC) {
+                int pb = p_base[0], pw = p_w[0], pr = p_row[0], pc = p_ca[0];
+#pragma unroll
+                for (int q = 1; q < kMaxParts; ++q)
+                    if (f >= p_base[q] && p_w[q] > 0) { pb = p_base[q]; pw = p_w[q]; pr = p_row[q]; pc = p_ca[q]; }
+                const int r = (f - pb) / pw, j = (f - pb) - r * pw;
+                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
+                const int cy = pr % nx, cz = DIM == 3 ? pr / nx : 0;
+                const int x = wrap_cell(pc - 1 + j, nx);
+                const int y = wrap_cell(cy + oy, nx);
+                const int z = DIM == 3 ? wrap_cell(cz + oz, nx) : 0;
+                const uint32_t ci = cb + (uint32_t)(x + y * nx + z * nx * nx);
+                const uint32_t s0 = a.cell_start[ci];
+                c = a.cell_end[ci] - s0;
+                gs[f] = s0;
+            }
+            const uint32_t inc = wave_incl_scan(c);
+            if (lane == kWave - 1) s_cnt[wv] = inc;
+            __syncthreads();
+            uint32_t wbase0 = carry, tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                wbase0 += w < wv ? s_cnt[w] : 0u;
+                tot += s_cnt[w];
+            }
+            if (f < NC) off[f] = wbase0 + inc - c;
+            carry += tot;
+            __syncthreads();
+        }
+        if (tid == 0) off[NC] = carry;
+    }
+    __syncthreads();
+
+    // Balanced path: the whole staged neighbourhood fits one window and every
+    // home's candidate runs fit its mask words (a run of len candidates takes
+    // ceil(len / 32) words; known here from the staged offsets).  Homes are
+    // then screened in their sorted order and stepped, after a deal by their
+    // kept-candidate counts, by lanes of nearly equal work.
+    const int S = (int)carry;
+    bool bal = false;
+    if (use_lds && a.wcap > 0 && S <= a.cap) {
+        int need = 0;
+        if (active0) {
+            int pb = p_base[0], pw = p_w[0], pc = p_ca[0];
+#pragma unroll
+            for (int q = 1; q < kMaxParts; ++q)
+                if (mypart0 == q) { pb = p_base[q]; pw = p_w[q]; pc = p_ca[q]; }
+            const int jh = hcx0 - pc + 1;
+            for (int r = 0; r < NR; ++r) {
+                const int rb = pb + r * pw;
+                need += ((int)(off[rb + jh + 2] - off[rb + jh - 1]) + 31) >> 5;
+            }
+        }
+        need = wave_max(need);
+        if (lane == 0) s_cnt[wv] = (uint32_t)need;
+        __syncthreads();
+        int mx = 0;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) mx = max(mx, (int)s_cnt[v]);
+        bal = mx <= a.wcap;
+        __syncthreads();
+    }
+
+    // Windowed path: deal the homes to threads grouped by where they sit in
+    // their cell (y quartile in 2D, y/z halves in 3D; stable within a group).
+    // A home's accepted candidates bunch in the stencil rows nearest to it, so
+    // lanes of one wave then fill their candidate lists at similar rates and
+    // list flushes keep more lanes busy.  Each home's own work is unchanged.
+    // The balanced path screens home tid on thread tid.
+    uint32_t dealt = (uint32_t)tid | ((uint32_t)max(0, min(mypart0, 255)) << 16);
+    if (!bal) {
+        const int dkey = active0 ? deal_key<DIM>(a.sy[a.perm[gs0]], DIM == 3 ? a.sz[a.perm[gs0]] : 0u,
+                                                 hrow0, nx, w.cs)
+                                 : kDealKeys - 1;
+        const uint64_t lt = (1ull << lane) - 1;
+        uint32_t myrank = 0;
+#pragma unroll
+        for (int q = 0; q < kDealKeys; ++q) {
+            const uint64_t b = __ballot(dkey == q);
+            if (lane == 0) s_qc[wv][q] = (uint32_t)__popcll(b);
+            if (dkey == q) myrank = (uint32_t)__popcll(b & lt);
+        }
+        __syncthreads();
+        for (int q = 0; q < kDealKeys; ++q)
+#pragma unroll
+            for (int v = 0; v < NW; ++v)
+                myrank += (q < dkey || (q == dkey && v < wv)) ? s_qc[v][q] : 0u;
+        s_deal[myrank] = dealt;
+        __syncthreads();
+        dealt = s_deal[tid];
+    }
+
+    // this thread's home (after the deal)
+    const int th = (int)(dealt & 0xFFFFu);
+    const int mypart = (int)(dealt >> 16);
+    const bool active = th < nact;
+    const int g = g0 + (active ? th : nact - 1);
+    const uint32_t hkey = a.skeys[g] - cb;
+    const int hrow = (int)(hkey / (uint32_t)nx);
+    const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
+    const uint32_t ho = a.perm[g];
+    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
+    const int htype = a.type[ho];
+    const int mrow = htype * w.ntypes;
 
     // 3D phase A screens on 16-bit positions relative to the workgroup's first
     // home: rel = (int32)(p - origin) >> sh, with sh chosen so that every
@@ -528,50 +621,52 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         bq = (int)fmin(2147483647.0, floor(rq * rq) + 1.0);
     }
 
-    // staged cells: start and count, then an exclusive prefix in LDS (carry ends
-    // as the staged particle count, the same in every thread)
-    uint32_t carry = 0;
-    if (use_lds) {
-        for (int f0 = 0; f0 < NC; f0 += BS) {
-            const int f = f0 + tid;
-            uint32_t c = 0;
-            if (f < NC) {
-                int pb = p_base[0], pw = p_w[0], pr = p_row[0], pc = p_ca[0];
-#pragma unroll
-                for (int q = 1; q < kMaxParts; ++q)
-                    if (f >= p_base[q] && p_w[q] > 0) { pb = p_base[q]; pw = p_w[q]; pr = p_row[q]; pc = p_ca[q]; }
-                const int r = (f - pb) / pw, j = (f - pb) - r * pw;
-                const int oy = (r % 3) - 1, oz = DIM == 3 ? (r / 3) - 1 : 0;
-                const int cy = pr % nx, cz = DIM == 3 ? pr / nx : 0;
-                const int x = wrap_cell(pc - 1 + j, nx);
-                const int y = wrap_cell(cy + oy, nx);
-                const int z = DIM == 3 ? wrap_cell(cz + oz, nx) : 0;
-                const uint32_t ci = cb + (uint32_t)(x + y * nx + z * nx * nx);
-                const uint32_t s0 = a.cell_start[ci];
-                c = a.cell_end[ci] - s0;
-                gs[f] = s0;
-            }
-            const uint32_t inc = wave_incl_scan(c);
-            if (lane == kWave - 1) s_cnt[wv] = inc;
-            __syncthreads();
-            uint32_t wbase = carry, tot = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                wbase += w < wv ? s_cnt[w] : 0u;
-                tot += s_cnt[w];
-            }
-            if (f < NC) off[f] = wbase + inc - c;
-            carry += tot;
-            __syncthreads();
+    // One candidate k of the staged window for a home at (px, py, pz) with
+    // matrix row prow_m: the reference's FP64 predicate and force law.
+    auto visit = [&](Acc &ac, int k, uint32_t px, uint32_t py, uint32_t pz, int prow_m) {
+        const uint2 q = sxy[k];
+        const int dxi = (int)(q.x - px), dyi = (int)(q.y - py);
+        const int dzi = DIM == 3 ? (int)(szz[k] - pz) : 0;
+        if constexpr (!WRAP) {
+            // plain difference of the clamped world: |D| >= 1/2 unless the
+            // int32 difference has the sign of the unsigned order
+            bool ok = ((dxi >= 0) == (q.x >= px)) & ((dyi >= 0) == (q.y >= py));
+            if (DIM == 3) ok &= ((dzi >= 0) == (szz[k] >= pz));
+            if (!ok) return;
         }
-        if (tid == 0) off[NC] = carry;
-    }
-    __syncthreads();
+        const int idx = prow_m + (int)st[k];
+        const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
+        pair64<DIM, DEBUG>(ac, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
+    };
+    // Stages window [w0, w0 + wn) of the staged neighbourhood.
+    auto stage = [&](int w0, int wn) {
+        for (int s = tid; s < wn; s += BS) {
+            const int sg = w0 + s;
+            int lo = 0, hi = NC;           // off[lo] <= sg < off[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (off[mid] <= (uint32_t)sg) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t q = a.perm[gs[lo] + ((uint32_t)sg - off[lo])];
+            const uint32_t qx = a.sx[q], qy = a.sy[q];
+            sxy[s] = make_uint2(qx, qy);
+            if constexpr (DIM == 3) {
+                const uint32_t qz = a.sz[q];
+                szz[s] = qz;
+                sq[s] = rel16(qx, ox, sh) | (rel16(qy, oy, sh) << 16);
+                sqz[s] = (uint16_t)rel16(qz, oz, sh);
+            }
+            if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
+            st[s] = (uint8_t)a.type[q];
+        }
+    };
 
     Acc acc;
     acc.x = acc.y = acc.z = 0.0;
     acc.cnt = 0;
     acc.hash = 0ull;
+    unsigned long long mycnt = 0;      // interacting pairs of the homes this thread finished
     if (use_lds) {
         int pb = p_base[0], pw = p_w[0], pc = p_ca[0];
 #pragma unroll
@@ -590,119 +685,208 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         const uint32_t hq = rel16(hx, ox, sh) | (rel16(hy, oy, sh) << 16);
         const uint32_t hqz = DIM == 3 ? rel16(hz, oz, sh) : 0u;
 
-        // The staged neighbourhood is S particles; it is brought into LDS in
-        // windows of at most cap (one window for ordinary densities, several
-        // for clustered cells) and every lane scans its ranges clipped to it.
-        const int S = (int)carry;
-        for (int w0 = 0; w0 < S; w0 += a.cap) {
-            const int wn = min(a.cap, S - w0);
-            for (int s = tid; s < wn; s += BS) {
-                const int sg = w0 + s;
-                int lo = 0, hi = NC;           // off[lo] <= sg < off[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (off[mid] <= (uint32_t)sg) lo = mid;
-                    else hi = mid;
-                }
-                const uint32_t q = a.perm[gs[lo] + ((uint32_t)sg - off[lo])];
-                const uint32_t qx = a.sx[q], qy = a.sy[q];
-                sxy[s] = make_uint2(qx, qy);
-                if constexpr (DIM == 3) {
-                    const uint32_t qz = a.sz[q];
-                    szz[s] = qz;
-                    sq[s] = rel16(qx, ox, sh) | (rel16(qy, oy, sh) << 16);
-                    sqz[s] = (uint16_t)rel16(qz, oz, sh);
-                }
-                if constexpr (DEBUG) sid[s] = (uint32_t)a.id[q];
-                st[s] = (uint8_t)a.type[q];
+        // U candidates from base kb of a row run of len (lane-local offset i):
+        // bit u set when candidate kb + u passes the phase-A screen
+        auto screen_group = [&](auto narrow_tag, int kb, int i, int len) -> uint32_t {
+            constexpr bool NARROW = decltype(narrow_tag)::value;
+            bool pass[U];
+            if constexpr (NARROW) {
+                const uint32_t *qp = sq + kb;
+                const uint16_t *qzp = sqz + kb;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    pass[u] = screen16<DIM>(qp[u], hq, DIM == 3 ? (uint32_t)qzp[u] : 0u, hqz, bq);
+            } else {
+                const uint2 *qp = sxy + kb;
+                const uint32_t *qzp = szz + kb;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    pass[u] = screen<DIM, WRAP>(hx, hy, hz, qp[u].x, qp[u].y, DIM == 3 ? qzp[u] : 0u,
+                                                w.r2hi);
             }
-            __syncthreads();
+            uint32_t keep = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) keep |= (pass[u] && i + u < len) ? (1u << u) : 0u;
+            return keep;
+        };
 
-            // Per-lane candidate list: entries at lists[j * BS + tid]; lp walks
-            // the lane's column of the list in steps of BS entries.
-            uint16_t *const lbase = lists + tid;
-            uint16_t *lp = lbase;
-            uint16_t *const lfull = lbase + (a.lcap - U) * BS;     // flush beyond this
-            auto flush = [&]() {
-                for (uint16_t *e = lbase; e < lp; e += BS) {
-                    const int k = *e;
-                    const uint2 q = sxy[k];
-                    const int dxi = (int)(q.x - hx), dyi = (int)(q.y - hy);
-                    const int dzi = DIM == 3 ? (int)(szz[k] - hz) : 0;
-                    if constexpr (!WRAP) {
-                        // plain difference of the clamped world: |D| >= 1/2 unless the
-                        // int32 difference has the sign of the unsigned order
-                        bool ok = ((dxi >= 0) == (q.x >= hx)) & ((dyi >= 0) == (q.y >= hy));
-                        if (DIM == 3) ok &= ((dzi >= 0) == (szz[k] >= hz));
-                        if (!ok) continue;
-                    }
-                    const int idx = mrow + (int)st[k];
-                    const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
-                    pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
-                }
-                lp = lbase;
-            };
-            // the row scan, instantiated for the 16-bit and the 32-bit screen
-            auto scan_rows = [&](auto narrow_tag) {
-                constexpr bool NARROW = decltype(narrow_tag)::value;
+        if (bal) {
+            // ---- balanced path: one window -------------------------------
+            stage(0, S);
+            __syncthreads();
+            // Phase A: thread tid screens home tid; each 32 candidates of a row
+            // run give one mask word, stored (with its first candidate's index)
+            // when any candidate passed.
+            int nwd = 0, npass = 0;
+            auto scan_words = [&](auto narrow_tag) {
 #pragma unroll 1
                 for (int r = 0; r < NR; ++r) {
                     const int oyr = (r % 3) - 1, ozr = DIM == 3 ? (r / 3) - 1 : 0;
                     const bool skip = !active || (q2y && oyr == -1) || (q2z && ozr == -1);
                     const int rb = pb + r * pw;
-                    const int klo = max((int)off[rb + j_lo], w0) - w0;
-                    const int khi = min((int)off[rb + jh + 2], w0 + wn) - w0;
-                    const int len = skip ? 0 : max(0, khi - klo);
+                    const int klo = (int)off[rb + j_lo];
+                    const int len = skip ? 0 : max(0, (int)off[rb + jh + 2] - klo);
                     const int K = wave_max(len);
-                    for (int i = 0; i < K; i += U) {
-                        if (__any(lp > lfull)) flush();
-                        // U candidates from one base: a lane past its range reads the
-                        // window's first entries instead (and keeps none of them)
-                        const int kb = i < len ? klo + i : 0;
-                        bool pass[U];
-                        if constexpr (NARROW) {
-                            const uint32_t *qp = sq + kb;
-                            const uint16_t *qzp = sqz + kb;
+                    for (int i = 0; i < K; i += 4 * U) {
+                        uint32_t word = 0;
 #pragma unroll
-                            for (int u = 0; u < U; ++u)
-                                pass[u] = screen16<DIM>(qp[u], hq, DIM == 3 ? (uint32_t)qzp[u] : 0u, hqz, bq);
-                        } else {
-                            const uint2 *qp = sxy + kb;
-                            const uint32_t *qzp = szz + kb;
-#pragma unroll
-                            for (int u = 0; u < U; ++u)
-                                pass[u] = screen<DIM, WRAP>(hx, hy, hz, qp[u].x, qp[u].y,
-                                                            DIM == 3 ? qzp[u] : 0u, w.r2hi);
+                        for (int gq = 0; gq < 4; ++gq) {
+                            const int ig = i + gq * U;
+                            if (ig < K) {
+                                // a lane past its run reads the window's first entries
+                                // (and keeps none of them)
+                                const int kb = ig < len ? klo + ig : 0;
+                                word |= screen_group(narrow_tag, kb, ig, len) << (gq * U);
+                            }
                         }
-                        // branch-free append: every candidate is written to the next
-                        // free slot, which only a kept one claims (the flush check
-                        // above leaves room for U slots)
-#pragma unroll
-                        for (int u = 0; u < U; ++u) {
-                            *lp = (uint16_t)(kb + u);
-                            lp += (pass[u] && i + u < len) ? BS : 0;
+                        if (word) {
+                            wmask[nwd * BS + tid] = word;
+                            wbase[nwd * BS + tid] = (uint16_t)(klo + i);
+                            ++nwd;
+                            npass += __popc(word);
                         }
                     }
                 }
             };
-            // 2D screens on the 32-bit differences: its FP32 screen is short and
-            // the 16-bit copy would cost LDS occupancy; 3D gains 2/3 of its screen
             if constexpr (DIM == 3) {
-                if (narrow) scan_rows(std::true_type{});
-                else scan_rows(std::false_type{});
+                if (narrow) scan_words(std::true_type{});
+                else scan_words(std::false_type{});
             } else {
-                scan_rows(std::false_type{});
+                scan_words(std::false_type{});
             }
-            flush();
+            int si = 0;
+            if (active) {
+                const int f = pb + RMID * pw + jh;
+                si = (int)off[f] + (g - (int)gs[f]);
+            }
+            s_si[tid] = (uint16_t)si;
+            s_hc[tid] = (uint16_t)(active ? npass : 0);
+            // Deal the homes by their kept-candidate counts (a counting sort;
+            // the order inside a count does not matter): lane tid of the
+            // workgroup steps the home of rank tid, so the lanes of a wave
+            // carry nearly equal work.  Each home is stepped whole by one lane
+            // in its scan order, so its sums are those of the windowed path.
+            for (int t = tid; t < 256; t += BS) s_bin[t] = 0u;
             __syncthreads();
+            const int cbin = min(active ? npass : 0, 255);
+            const uint32_t local = atomicAdd(&s_bin[cbin], 1u);
+            __syncthreads();
+            {
+                uint32_t v[PER], sum = 0, total = 0;
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    v[q] = s_bin[tid * PER + q];
+                    sum += v[q];
+                }
+                uint32_t ex = block_excl_scan<BS>(sum, s_cnt, total);
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    s_bin[tid * PER + q] = ex;
+                    ex += v[q];
+                }
+            }
+            __syncthreads();
+            s_deal[s_bin[cbin] + local] = (uint32_t)tid;
+            __syncthreads();
+
+            // Phase B: the home of rank tid, its kept candidates word by word
+            const int h = (int)s_deal[tid];
+            if (h < nact) {
+                const int hsi = s_si[h];
+                const uint2 hp = sxy[hsi];
+                const uint32_t bhz = DIM == 3 ? szz[hsi] : 0u;
+                const int btype = (int)st[hsi];
+                const int bmrow = btype * w.ntypes;
+                Acc ab;
+                ab.x = ab.y = ab.z = 0.0;
+                ab.cnt = 0;
+                ab.hash = 0ull;
+                int left = s_hc[h], j = 0, kb = 0;
+                uint32_t m = 0;
+                while (left > 0) {
+                    if (m == 0) {
+                        m = wmask[j * BS + h];
+                        kb = wbase[j * BS + h];
+                        ++j;
+                    }
+                    const int k = kb + __builtin_ctz(m);
+                    m &= m - 1u;
+                    --left;
+                    visit(ab, k, hp.x, hp.y, bhz, bmrow);
+                }
+                const int gh = g0 + h;
+                finish_home<DIM, WRAP, DEBUG>(a, w, ab, hp.x, hp.y, bhz, btype, a.perm[gh], gh);
+                mycnt = (unsigned long long)ab.cnt;
+            }
+        } else {
+            // ---- windowed path --------------------------------------------
+            // The staged neighbourhood is S particles; it is brought into LDS in
+            // windows of at most cap (one window for ordinary densities, several
+            // for clustered cells) and every lane scans its ranges clipped to it.
+            for (int w0 = 0; w0 < S; w0 += a.cap) {
+                const int wn = min(a.cap, S - w0);
+                stage(w0, wn);
+                __syncthreads();
+
+                // Per-lane candidate list: entries at lists[j * BS + tid]; lp walks
+                // the lane's column of the list in steps of BS entries.
+                uint16_t *const lbase = lists + tid;
+                uint16_t *lp = lbase;
+                uint16_t *const lfull = lbase + (a.lcap - U) * BS;     // flush beyond this
+                auto flush = [&]() {
+                    for (uint16_t *e = lbase; e < lp; e += BS) visit(acc, *e, hx, hy, hz, mrow);
+                    lp = lbase;
+                };
+                // the row scan, instantiated for the 16-bit and the 32-bit screen
+                auto scan_rows = [&](auto narrow_tag) {
+#pragma unroll 1
+                    for (int r = 0; r < NR; ++r) {
+                        const int oyr = (r % 3) - 1, ozr = DIM == 3 ? (r / 3) - 1 : 0;
+                        const bool skip = !active || (q2y && oyr == -1) || (q2z && ozr == -1);
+                        const int rb = pb + r * pw;
+                        const int klo = max((int)off[rb + j_lo], w0) - w0;
+                        const int khi = min((int)off[rb + jh + 2], w0 + wn) - w0;
+                        const int len = skip ? 0 : max(0, khi - klo);
+                        const int K = wave_max(len);
+                        for (int i = 0; i < K; i += U) {
+                            if (__any(lp > lfull)) flush();
+                            // U candidates from one base: a lane past its range reads the
+                            // window's first entries instead (and keeps none of them)
+                            const int kb = i < len ? klo + i : 0;
+                            const uint32_t keep = screen_group(narrow_tag, kb, i, len);
+                            // branch-free append: every candidate is written to the next
+                            // free slot, which only a kept one claims (the flush check
+                            // above leaves room for U slots)
+#pragma unroll
+                            for (int u = 0; u < U; ++u) {
+                                *lp = (uint16_t)(kb + u);
+                                lp += ((keep >> u) & 1u) ? BS : 0;
+                            }
+                        }
+                    }
+                };
+                // 2D screens on the 32-bit differences: its FP32 screen is short and
+                // the 16-bit copy would cost LDS occupancy; 3D gains 2/3 of its screen
+                if constexpr (DIM == 3) {
+                    if (narrow) scan_rows(std::true_type{});
+                    else scan_rows(std::false_type{});
+                } else {
+                    scan_rows(std::false_type{});
+                }
+                flush();
+                __syncthreads();
+            }
         }
     } else if (active) {
         home_global<DIM, WRAP, DEBUG, MLDS>(a, w, M, g, hx, hy, hz, mrow, acc);
     }
-    if (active) finish_home<DIM, WRAP, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, g);
+    if (!bal && active) {
+        finish_home<DIM, WRAP, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, g);
+        mycnt = (unsigned long long)acc.cnt;
+    }
 
     // interacting pairs, one 64-bit atomic per workgroup
-    unsigned long long tot = active ? (unsigned long long)acc.cnt : 0ull;
+    unsigned long long tot = mycnt;
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) tot += __shfl_xor(tot, d, kWave);
     if (lane == 0) s_pairs[wv] = tot;

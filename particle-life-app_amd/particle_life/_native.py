@@ -101,6 +101,7 @@ class PlifeTuning(ctypes.Structure):
         ("direct_launch", ctypes.c_int32),
         ("stage_slack", ctypes.c_double),
         ("sparse_worlds", ctypes.c_int32),
+        ("windowed_only", ctypes.c_int32),
     ]
 
 

@@ -152,7 +152,8 @@ def test_launch_plans_bit_identical(eng):
     plans = [dict(), dict(block_homes=64, list_cap=8, stage_slack=0.05),
              dict(block_homes=128, list_cap=64, stage_slack=4.0),
              dict(block_homes=256, list_cap=16, stage_slack=0.3, direct_launch=True),
-             dict(sparse_worlds=2), dict(sparse_worlds=1, block_homes=64)]
+             dict(sparse_worlds=2), dict(sparse_worlds=1, block_homes=64),
+             dict(windowed_only=True), dict(windowed_only=True, block_homes=128)]
     ref = None
     for p in plans:
         eng.set_tuning(**p)
@@ -167,6 +168,32 @@ def test_launch_plans_bit_identical(eng):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=f"plan {p}")
         assert got[3] == ref[3]
     eng.set_tuning()
+
+
+@pytest.mark.parametrize("dim,wrap", [(2, True), (2, False), (3, True), (3, False)])
+def test_balanced_path_equals_windowed(eng, dim, wrap):
+    """Uniform worlds step on the balanced path (one staging window, homes
+    dealt to lanes by their kept-candidate counts) bit-identically to the
+    windowed path with per-thread lists, neighbour sets included; both
+    against the oracle."""
+    print(f"\n[parity] case=balanced_vs_windowed dim={dim} wrap={wrap}", flush=True)
+    w = _world(160 + dim + 2 * wrap, 40_000 if dim == 2 else 30_000, 7,
+               0.012 if dim == 2 else 0.05, 0.01, dim=dim, wrap=wrap)
+    out = []
+    try:
+        for win in (False, True):
+            eng.set_tuning(windowed_only=win)
+            eng.load_world(w)
+            check_one_step(eng, w)
+            eng.step(3)
+            check_one_step(eng, w)
+            st = eng.state()
+            cnt, hsh = eng.neighbor_summary()
+            out.append((st["pos"], st["vel"], st["ids"], cnt, hsh))
+    finally:
+        eng.set_tuning()
+    for k in range(5):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
 
 
 SPARSE = {
