@@ -73,7 +73,7 @@ constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9
 constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
-    int wcap = 0;                     // mask words per home of the balanced path (0: windowed only)
+    int wpool = 0;                    // mask words per workgroup of the balanced path (0: windowed only)
     int bs, lcap, max_cells, cap;
     size_t lds;
 };
@@ -456,6 +456,8 @@ WorldDev world_dev(const WorldHost &w, uint32_t cell_base, int moff)
     d.one_plus_beta = 1.0 + s.beta;
     d.inv_1mb = 1.0 / (1.0 - s.beta);
     d.inv_beta = 1.0 / s.beta;
+    d.m2S = -2.0 * d.S;
+    d.betaR = s.beta * R;
     d.ff = std::pow(s.friction, 60 * s.dt);            // Physics.java:401
     d.ka = s.force * kInvFix;
     d.kdt = s.force * s.dt / kFix;
@@ -920,13 +922,13 @@ ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const 
     // few sparse workgroups that span more cells take the global path.
     const double span = p.bs / std::max(avg, 1e-9) + 1.0;
     // balanced path (uniform worlds): a home's stencil row is a run of three
-    // cells, ~3 avg candidates; its mask words must hold the longest run of a
-    // workgroup (workgroups where one does not fit take the windowed path)
+    // cells, ~3 avg candidates, ceil(run / 32) mask words; the workgroup's
+    // pool holds 10% more than the expected total (workgroups needing more
+    // take the windowed path)
     if (!clustered && !t.windowed_only) {
         const double run = 3.0 * avg;
-        const double run_hi = run + 4.0 * std::sqrt(run) + 3.0;
-        const int wcap = NR * (int)std::ceil(run_hi / 32.0);
-        if ((size_t)wcap * p.bs * 6 <= kWordLds) p.wcap = wcap;
+        const long long pool = (long long)std::ceil(p.bs * NR * (run / 32.0 + 0.5) * 1.1) + 64;
+        if (pool <= 65535 && (size_t)pool * 6 <= kWordLds) p.wpool = (int)pool;
     }
     p.max_cells = (int)std::min<double>(std::max(64.0, std::ceil(NR * (span + 2.0) * 1.5 + 16)),
                                         clustered ? 384.0 : 2048.0);
@@ -943,7 +945,7 @@ ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const 
     const bool m_in_lds = ntypes_max <= kMaxTypesLds;
     auto total = [&](long long c) {
         return force_lds_layout(dim, dbg, ntypes_max, m_in_lds, (int)c, p.lcap, p.bs,
-                                p.max_cells, p.wcap).total;
+                                p.max_cells, p.wpool).total;
     };
     // keep room for a useful staging window: shrink the cell table first
     const long long min_cap = std::min<long long>(want, 256);
@@ -1227,7 +1229,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     a.cap = fp.cap;
     a.lcap = fp.lcap;
     a.max_cells = fp.max_cells;
-    a.wcap = fp.wcap;
+    a.wpool = fp.wpool;
     a.pair_total = h->d_pairs;
     a.pair_world = h->d_pairs + 2;
     a.block_work = h->work_dev;
@@ -2192,7 +2194,7 @@ int noalloc_step(const plife_settings *s, int32_t ntypes, const double *matrix, 
         a.cap = fp.cap;
         a.lcap = fp.lcap;
         a.max_cells = fp.max_cells;
-        a.wcap = fp.wcap;
+        a.wpool = fp.wpool;
         a.pair_total = t.d_pairs;
         a.pair_world = t.d_pairs + 2;
         a.block_work = work;

@@ -47,7 +47,7 @@ struct ForceArgs {
     int block_base;                  // this launch's first entry of blocks
     int ntypes_max, m_in_lds;
     int cap, lcap, max_cells;
-    int wcap;                        // mask words per home of the balanced path (0: windowed only)
+    int wpool;                       // mask words per workgroup of the balanced path (0: windowed only)
     unsigned long long *pair_total;  // interacting pairs of the step, all worlds
     unsigned long long *pair_world;  // per world
     uint32_t *block_work;            // pinned host memory: per table entry its pairs (saturated); [nb_total] = tag
@@ -78,8 +78,11 @@ __device__ __forceinline__ unsigned long long id_hash(uint32_t id)
 //   f/dist = 1/beta - 1/dist                             (dist <  beta)
 //          = a*(1 - |1 + beta - 2 dist|/(1 - beta))/dist (dist >= beta)
 // (app/Main.java:275-280).  acc accumulates D*(f/dist) in units of 2^-32.
+// With u = |D| = dist R (R = rmax 2^32) and y = 1/|D|: 1/dist = R y, and the
+// attractive branch is (a R) (1 - |1 + beta - 2 S u|/(1 - beta)) y, so the
+// caller passes mR = a * R (staged that way with the matrix).
 template <int DIM, bool DEBUG>
-__device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, double m, uint32_t qid,
+__device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, double mR, uint32_t qid,
                                        const WorldDev &a)
 {
     const double dx = (double)dxi, dy = (double)dyi, dz = (double)dzi;
@@ -95,13 +98,12 @@ __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, doub
         const double e = d2 * y0;
         const double h = fma(-e, y0, 1.0);
         const double y = fma(0.5 * y0, h, y0);        // 1/|D|, one Newton step from FP32
-        const double dist = (d2 * y) * a.S;
-        const double inv = y * a.R;                    // 1/dist
-        const double t = fabs(fma(dist, -2.0, a.one_plus_beta));
+        const double u = d2 * y;                       // |D| = dist R
+        const double t = fabs(fma(u, a.m2S, a.one_plus_beta));
         const double g = fma(t, -a.inv_1mb, 1.0);
-        const double att = (m * g) * inv;
-        const double rep = a.inv_beta - inv;
-        const double c = dist < a.beta ? rep : att;
+        const double att = (mR * g) * y;
+        const double rep = fma(-a.R, y, a.inv_beta);   // 1/beta - 1/dist
+        const double c = u < a.betaR ? rep : att;
         acc.x = fma(c, dx, acc.x);
         acc.y = fma(c, dy, acc.y);
         if constexpr (DIM == 3) acc.z = fma(c, dz, acc.z);
@@ -128,30 +130,22 @@ __device__ __forceinline__ int wave_max(int v)
 }
 
 // Phase A screen on exact fixed-point deltas.  In a clamped world the int32
-// difference equals the plain difference only when its sign matches the
-// unsigned order; otherwise |D| >= 1/2 > rmax and the pair is dropped.
-template <int DIM, bool WRAP>
+// difference may wrap across the box (|D| >= 1/2 > rmax); phase B's exact
+// sign test drops those, so the screen stays a superset.
+template <int DIM>
 __device__ __forceinline__ bool screen(uint32_t hx, uint32_t hy, uint32_t hz, uint32_t qx,
                                        uint32_t qy, uint32_t qz, float r2hi)
 {
-    const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
-    const float dx = (float)dxi, dy = (float)dyi;
+    const float dx = (float)(int)(qx - hx), dy = (float)(int)(qy - hy);
     float d2 = dx * dx;
     d2 = fmaf(dy, dy, d2);
-    int dzi = 0;
     if constexpr (DIM == 3) {
-        dzi = (int)(qz - hz);
-        const float dz = (float)dzi;
+        const float dz = (float)(int)(qz - hz);
         d2 = fmaf(dz, dz, d2);
     }
     // d2 == 0 (the home itself, coincident particles) passes the screen; the
     // FP64 predicate of phase B drops it
-    bool pass = d2 <= r2hi;
-    if constexpr (!WRAP) {
-        pass &= ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
-        if constexpr (DIM == 3) pass &= ((dzi >= 0) == (qz >= hz));
-    }
-    return pass;
+    return d2 <= r2hi;
 }
 
 // Periodic distance in cells between columns (rows, layers) a and b of n.
@@ -263,7 +257,7 @@ __device__ __forceinline__ void visit_run(const ForceArgs &a, const WorldDev &w,
             if (!ok) continue;
         }
         const int idx = row + a.type[jo];
-        const double m = MLDS ? Mlds[idx] : a.matrix[w.moff + idx];
+        const double m = MLDS ? Mlds[idx] : a.matrix[w.moff + idx] * w.R;
         pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m, DEBUG ? (uint32_t)a.id[jo] : 0u, w);
     }
 }
@@ -348,11 +342,12 @@ struct ForceLds {
 };
 
 // The candidate lists (windowed path: lcap u16 entries per thread) and the
-// mask words (balanced path: wcap u32 masks + wcap u16 first indices per
-// thread) share one region; a workgroup takes one path.
+// mask words (balanced path: a pool of wpool u32 masks + wpool u16 first
+// indices, each home its own run of them) share one region; a workgroup takes
+// one path.
 __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int ntypes, bool m_in_lds,
                                                      int cap, int lcap, int bs, int max_cells,
-                                                     int wcap)
+                                                     int wpool)
 {
     ForceLds L;
     size_t o = 0;
@@ -362,9 +357,9 @@ __host__ __device__ inline ForceLds force_lds_layout(int dim, bool debug, int nt
     L.sxy = o;   o += (size_t)(cap + kScreenPad) * 8;
     L.sq = o;    o += dim == 3 ? (size_t)(cap + kScreenPad) * 4 : 0;     // 16-bit screen (3D only)
     L.lists = o;
-    L.wbase = o + (size_t)wcap * bs * 4;
+    L.wbase = o + (size_t)wpool * 4;
     {
-        const size_t lsz = (size_t)lcap * bs * 2, wsz = (size_t)wcap * bs * 6;
+        const size_t lsz = (size_t)lcap * bs * 2, wsz = (size_t)wpool * 6;
         o += lsz > wsz ? lsz : wsz;
     }
     o = (o + 3) & ~(size_t)3;
@@ -397,11 +392,11 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     __shared__ uint32_t s_qc[NW][kDealKeys];
     __shared__ uint32_t s_deal[BS];
     // balanced path: per home its staged index, kept candidates and mask words
-    __shared__ uint16_t s_si[BS], s_hc[BS];
+    __shared__ uint16_t s_si[BS], s_hc[BS], s_wo[BS];
     __shared__ uint32_t s_bin[256];
 
     const ForceLds L = force_lds_layout(DIM, DEBUG, a.ntypes_max, MLDS, a.cap, a.lcap, BS, a.max_cells,
-                                        a.wcap);
+                                        a.wpool);
     double *M = reinterpret_cast<double *>(smem + L.m);
     uint2 *sxy = reinterpret_cast<uint2 *>(smem + L.sxy);
     uint32_t *sq = reinterpret_cast<uint32_t *>(smem + L.sq);
@@ -425,12 +420,19 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const int nact = min(BS, w.off + w.n - g0);
 
     if constexpr (MLDS)
-        for (int t = tid; t < w.ntypes * w.ntypes; t += BS) M[t] = a.matrix[w.moff + t];
+        for (int t = tid; t < w.ntypes * w.ntypes; t += BS) M[t] = a.matrix[w.moff + t] * w.R;
 
     // Thread tid first stands for home g0 + tid (sorted order), to find the
     // parts: grid rows (2D: cy; 3D: cy + cz*nx) and columns of the homes.
+    // Its position and type, and the origin's (home g0) position, are read
+    // up front: independent loads whose latency overlaps the setup.
     const bool active0 = tid < nact;
     const int gs0 = g0 + (active0 ? tid : nact - 1);
+    const uint32_t ho0 = a.perm[gs0];
+    const uint32_t hx0 = a.sx[ho0], hy0 = a.sy[ho0], hz0 = DIM == 3 ? a.sz[ho0] : 0u;
+    const int htype0 = a.type[ho0];
+    const uint32_t go = a.perm[g0];
+    const uint32_t ox = a.sx[go], oy = a.sy[go], oz = DIM == 3 ? a.sz[go] : 0u;
     const uint32_t hkey0 = a.skeys[gs0] - cb;
     const int hrow0 = (int)(hkey0 / (uint32_t)nx);
     const int hcx0 = (int)(hkey0 - (uint32_t)hrow0 * (uint32_t)nx);
@@ -513,14 +515,15 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     }
     __syncthreads();
 
-    // Balanced path: the whole staged neighbourhood fits one window and every
-    // home's candidate runs fit its mask words (a run of len candidates takes
-    // ceil(len / 32) words; known here from the staged offsets).  Homes are
-    // then screened in their sorted order and stepped, after a deal by their
+    // Balanced path: the whole staged neighbourhood fits one window and the
+    // homes' candidate runs fit the pool of mask words (a run of len
+    // candidates takes ceil(len / 32) words; known here from the staged
+    // offsets, so each home gets its own run of the pool).  Homes are then
+    // screened in their sorted order and stepped, after a deal by their
     // kept-candidate counts, by lanes of nearly equal work.
     const int S = (int)carry;
     bool bal = false;
-    if (use_lds && a.wcap > 0 && S <= a.cap) {
+    if (use_lds && a.wpool > 0 && S <= a.cap) {
         int need = 0;
         if (active0) {
             int pb = p_base[0], pw = p_w[0], pc = p_ca[0];
@@ -533,14 +536,10 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                 need += ((int)(off[rb + jh + 2] - off[rb + jh - 1]) + 31) >> 5;
             }
         }
-        need = wave_max(need);
-        if (lane == 0) s_cnt[wv] = (uint32_t)need;
-        __syncthreads();
-        int mx = 0;
-#pragma unroll
-        for (int v = 0; v < NW; ++v) mx = max(mx, (int)s_cnt[v]);
-        bal = mx <= a.wcap;
-        __syncthreads();
+        uint32_t total = 0;
+        const uint32_t woff = block_excl_scan<BS>((uint32_t)need, s_cnt, total);
+        s_wo[tid] = (uint16_t)woff;
+        bal = total <= (uint32_t)a.wpool;
     }
 
     // Windowed path: deal the homes to threads grouped by where they sit in
@@ -551,9 +550,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     // The balanced path screens home tid on thread tid.
     uint32_t dealt = (uint32_t)tid | ((uint32_t)max(0, min(mypart0, 255)) << 16);
     if (!bal) {
-        const int dkey = active0 ? deal_key<DIM>(a.sy[a.perm[gs0]], DIM == 3 ? a.sz[a.perm[gs0]] : 0u,
-                                                 hrow0, nx, w.cs)
-                                 : kDealKeys - 1;
+        const int dkey = active0 ? deal_key<DIM>(hy0, hz0, hrow0, nx, w.cs) : kDealKeys - 1;
         const uint64_t lt = (1ull << lane) - 1;
         uint32_t myrank = 0;
 #pragma unroll
@@ -577,12 +574,14 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     const int mypart = (int)(dealt >> 16);
     const bool active = th < nact;
     const int g = g0 + (active ? th : nact - 1);
-    const uint32_t hkey = a.skeys[g] - cb;
+    // (the balanced path keeps home tid: its data is already here)
+    const uint32_t hkey = bal ? hkey0 : a.skeys[g] - cb;
     const int hrow = (int)(hkey / (uint32_t)nx);
     const int hcx = (int)(hkey - (uint32_t)hrow * (uint32_t)nx);
-    const uint32_t ho = a.perm[g];
-    const uint32_t hx = a.sx[ho], hy = a.sy[ho], hz = DIM == 3 ? a.sz[ho] : 0u;
-    const int htype = a.type[ho];
+    const uint32_t ho = bal ? ho0 : a.perm[g];
+    const uint32_t hx = bal ? hx0 : a.sx[ho], hy = bal ? hy0 : a.sy[ho];
+    const uint32_t hz = DIM == 3 ? (bal ? hz0 : a.sz[ho]) : 0u;
+    const int htype = bal ? htype0 : a.type[ho];
     const int mrow = htype * w.ntypes;
 
     // 3D phase A screens on 16-bit positions relative to the workgroup's first
@@ -593,8 +592,6 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
     // lies within a quarter box of the origin (two such points are less than
     // half a box apart); wider workgroups (coarse grids, sparse parts) screen
     // on the exact 32-bit differences instead.
-    const uint32_t go = a.perm[g0];
-    const uint32_t ox = a.sx[go], oy = a.sy[go], oz = DIM == 3 ? a.sz[go] : 0u;
     int sh = 0, bq = 0;
     bool narrow = false;
     {
@@ -635,7 +632,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             if (!ok) return;
         }
         const int idx = prow_m + (int)st[k];
-        const double m = MLDS ? M[idx] : a.matrix[w.moff + idx];
+        const double m = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
         pair64<DIM, DEBUG>(ac, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
     };
     // Stages window [w0, w0 + wn) of the staged neighbourhood.
@@ -678,36 +675,39 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         // reference then scans columns {nx-1, 0, 1}: skip staged column nx-2
         // (column 1 is always farther than rmax).  Same for rows / layers.
         const int cy = hrow % nx, cz = DIM == 3 ? hrow / nx : 0;
-        const bool q2x = (hcx == nx - 1) && stencil_origin(hx, w.cs) == nx;
-        const bool q2y = (cy == nx - 1) && stencil_origin(hy, w.cs) == nx;
-        const bool q2z = DIM == 3 && (cz == nx - 1) && stencil_origin(hz, w.cs) == nx;
+        // (the FP64 division only for homes in a last column, row or layer)
+        bool q2x = false, q2y = false, q2z = false;
+        if (hcx == nx - 1) q2x = stencil_origin(hx, w.cs) == nx;
+        if (cy == nx - 1) q2y = stencil_origin(hy, w.cs) == nx;
+        if (DIM == 3 && cz == nx - 1) q2z = stencil_origin(hz, w.cs) == nx;
         const int j_lo = jh - 1 + (q2x ? 1 : 0);
         const uint32_t hq = rel16(hx, ox, sh) | (rel16(hy, oy, sh) << 16);
         const uint32_t hqz = DIM == 3 ? rel16(hz, oz, sh) : 0u;
 
         // U candidates from base kb of a row run of len (lane-local offset i):
         // bit u set when candidate kb + u passes the phase-A screen
+        // NARROW (3D): on the staged 16-bit copies; otherwise FP32 on the
+        // exact differences.  Bits at or past the run's end (i + u >= len)
+        // stay clear.
         auto screen_group = [&](auto narrow_tag, int kb, int i, int len) -> uint32_t {
             constexpr bool NARROW = decltype(narrow_tag)::value;
-            bool pass[U];
-            if constexpr (NARROW) {
+            uint32_t keep = 0;
+            if constexpr (NARROW && DIM == 3) {
                 const uint32_t *qp = sq + kb;
                 const uint16_t *qzp = sqz + kb;
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    pass[u] = screen16<DIM>(qp[u], hq, DIM == 3 ? (uint32_t)qzp[u] : 0u, hqz, bq);
+                    keep |= screen16<DIM>(qp[u], hq, (uint32_t)qzp[u], hqz, bq) ? (1u << u) : 0u;
             } else {
                 const uint2 *qp = sxy + kb;
                 const uint32_t *qzp = szz + kb;
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    pass[u] = screen<DIM, WRAP>(hx, hy, hz, qp[u].x, qp[u].y, DIM == 3 ? qzp[u] : 0u,
-                                                w.r2hi);
+                    keep |= screen<DIM>(hx, hy, hz, qp[u].x, qp[u].y, DIM == 3 ? qzp[u] : 0u, w.r2hi)
+                                ? (1u << u) : 0u;
             }
-            uint32_t keep = 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) keep |= (pass[u] && i + u < len) ? (1u << u) : 0u;
-            return keep;
+            const int nv = min(max(len - i, 0), U);
+            return keep & ((1u << nv) - 1u);
         };
 
         if (bal) {
@@ -718,6 +718,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
             // run give one mask word, stored (with its first candidate's index)
             // when any candidate passed.
             int nwd = 0, npass = 0;
+            const int wo = s_wo[tid];          // this home's run of the word pool
             auto scan_words = [&](auto narrow_tag) {
 #pragma unroll 1
                 for (int r = 0; r < NR; ++r) {
@@ -740,14 +741,16 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                             }
                         }
                         if (word) {
-                            wmask[nwd * BS + tid] = word;
-                            wbase[nwd * BS + tid] = (uint16_t)(klo + i);
+                            wmask[wo + nwd] = word;
+                            wbase[wo + nwd] = (uint16_t)(klo + i);
                             ++nwd;
                             npass += __popc(word);
                         }
                     }
                 }
             };
+            // 2D screens on the 32-bit differences: its FP32 screen is short;
+            // 3D gains 2/3 of its screen on the 16-bit copies
             if constexpr (DIM == 3) {
                 if (narrow) scan_words(std::true_type{});
                 else scan_words(std::false_type{});
@@ -801,12 +804,13 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                 ab.x = ab.y = ab.z = 0.0;
                 ab.cnt = 0;
                 ab.hash = 0ull;
-                int left = s_hc[h], j = 0, kb = 0;
+                int left = s_hc[h], j = s_wo[h], kb = 0;
                 uint32_t m = 0;
+#pragma unroll 1
                 while (left > 0) {
                     if (m == 0) {
-                        m = wmask[j * BS + h];
-                        kb = wbase[j * BS + h];
+                        m = wmask[j];
+                        kb = wbase[j];
                         ++j;
                     }
                     const int k = kb + __builtin_ctz(m);
@@ -865,8 +869,6 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                         }
                     }
                 };
-                // 2D screens on the 32-bit differences: its FP32 screen is short and
-                // the 16-bit copy would cost LDS occupancy; 3D gains 2/3 of its screen
                 if constexpr (DIM == 3) {
                     if (narrow) scan_rows(std::true_type{});
                     else scan_rows(std::false_type{});
@@ -924,7 +926,7 @@ __global__ __launch_bounds__(kSparseBlock) void k_force_sparse(ForceArgs a)
     const uint2 bw = a.blocks[a.block_base + blk];
     const WorldDev w = a.worlds[bw.x];
     if constexpr (MLDS) {
-        for (int t = tid; t < w.ntypes * w.ntypes; t += kSparseBlock) M[t] = a.matrix[w.moff + t];
+        for (int t = tid; t < w.ntypes * w.ntypes; t += kSparseBlock) M[t] = a.matrix[w.moff + t] * w.R;
         __syncthreads();
     }
     const int g = (int)bw.y + tid;

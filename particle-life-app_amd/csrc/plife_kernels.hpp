@@ -125,6 +125,8 @@ struct WorldDev {
     double S;                        // 2^-32 / rmax: |rel| = |D| S with D in units 2^-32
     double R;                        // rmax * 2^32:  1/|rel| = R / |D|
     double beta, one_plus_beta, inv_1mb, inv_beta;
+    double m2S;                      // -2 S
+    double betaR;                    // beta R: |D| below it is the repulsion zone
     double ff;                       // friction^(60 dt) (host FP64 pow, Physics.java:401)
     double kdt;                      // force * dt * 2^-32
     double ka;                       // force * 2^-32: acceleration per unit of accumulated D (f/dist)
