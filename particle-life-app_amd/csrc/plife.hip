@@ -1289,9 +1289,16 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     return v;
 }
 
+// Destroys the step graph.  A launch of it may still be executing on the
+// stream (updates return before the device finishes), so the stream is drained
+// first: the executable graph's resources are never freed under a running
+// launch.
 void drop_graph(plife_physics *h)
 {
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->gexec) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->gexec);
+    }
     h->gexec = nullptr;
     h->gkey.clear();
 }
