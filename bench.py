@@ -79,6 +79,17 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
+def force_kernel_sha256() -> str:
+    """Hash of the force kernel's sources (the build the counters belong to)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in ("plife_force.hpp", "plife_kernels.hpp"):
+        with open(os.path.join(ROOT, "particle-life-app_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def host_cpus() -> dict:
     """What the host offers the CPU baseline: logical CPUs, those this process
     may run on (affinity), the cgroup's CPU quota, and the CPU model."""
@@ -347,11 +358,20 @@ def main() -> int:
     force_s = prof["force_ms"] / 1e3
     achieved_tflops = FLOP_PER_PAIR[w.dim] * pairs / force_s / 1e12
 
-    traffic = None
+    # HBM bytes per force launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, used
+    # only when they were measured on this build of the force kernel
+    traffic, traffic_note = None, "not measured for this workload"
     tpath = os.path.join(ROOT, "profiles", "force_c2_traffic.json")
     if args.config == "c2" and not args.n and os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
+        if rec.get("kernel_source_sha256") == force_kernel_sha256():
+            traffic = rec.get("bytes_per_launch")
+            traffic_note = "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (scales from a 256 MiB copy), " \
+                           "profiles/force_c2_counters.md"
+        else:
+            traffic_note = "the recorded counters belong to another build of the force kernel"
+
 
     workloads = {
         "c1": "plain 2D world: N=10,000, 6 types, rmax 0.04 in [-1,1)^2 (0.02 unit box), beta 0.3, "
@@ -398,6 +418,7 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
             "traffic": traffic,
+            "traffic_note": traffic_note,
             "algorithmic": f"{FLOP_PER_PAIR[w.dim]} FLOP/pair x {pairs} pairs per launch",
             "hbm_frac_algorithmic": BYTES_PER_PARTICLE[w.dim] * w.n * steps_per_s / 1e9 / PEAK_HBM_GBS,
         },
