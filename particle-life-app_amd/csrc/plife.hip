@@ -240,6 +240,7 @@ struct plife_physics {
     size_t pairs_cap = 0;
     // pinned copy of d_pairs (3 + worlds entries), written by every step
     uint64_t *stat_host = nullptr;
+    unsigned long long *stat_dev = nullptr;   // the device's address of stat_host
     size_t stat_cap = 0;
     std::vector<void *> retired_host;  // replaced pinned buffers, freed at destroy
     unsigned char *pin = nullptr;      // pinned staging of state downloads
@@ -544,8 +545,15 @@ int upload_params(plife_physics *h, size_t extra)
         if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
         std::memset(p, 0, cap * 8);
         p[1] = h->stat_host ? h->stat_host[1] : 0;
+        unsigned long long *dp = nullptr;
+        const hipError_t e2 = hipHostGetDevicePointer((void **)&dp, p, 0);
+        if (e2 != hipSuccess) {
+            (void)hipHostFree(p);
+            return fail(PLIFE_ERR_DEVICE, "hipHostGetDevicePointer: %s", hipGetErrorString(e2));
+        }
         if (h->stat_host) h->retired_host.push_back(h->stat_host);
         h->stat_host = p;
+        h->stat_dev = dp;
         h->stat_cap = cap;
     }
     const size_t wb = W * sizeof(WorldDev), mb = h->matrices.size() * sizeof(double);
@@ -807,7 +815,7 @@ void launch_bin(plife_physics *h, int D, const PosBuf &P, int n, int ntiles)
     k_bin_tile<DIM, DD><<<ntiles, 1024, 0, h->stream>>>(P.x, P.y, P.z, n, h->d_worlds, wof,    \
                                                           ntiles, h->keys0, h->cell_count,     \
                                                           h->hist, h->d_pairs, ncounters,      \
-                                                          h->version + 1)
+                                                          h->version + 1, h->stat_dev)
     switch (D) {
         case 1: PLIFE_BIN(1); break;
         case 2: PLIFE_BIN(2); break;
@@ -1243,10 +1251,6 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     h->last_bs = fp.bs;
     h->last_lcap = fp.lcap;
     if ((rc = launch_force_any(h, a, fp.bs, fp.lds))) return rc;
-    // the step's counters (pairs, clustering measure, per-world pairs, layout
-    // tag) travel to pinned host memory for the launch planner
-    HIP_TRY(hipMemcpyAsync(h->stat_host, h->d_pairs, (3 + h->worlds.size()) * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, h->stream));
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
     h->ac = ao;
@@ -1332,6 +1336,17 @@ void count_steps(plife_physics *h, int k)
     for (WorldHost &w : h->worlds) w.steps += k;
 }
 
+// After the last step of an update: its counters to the host's pinned copy
+// for the launch planner (each step's bin kernel copies the previous step's).
+int copy_stats(plife_physics *h)
+{
+    if (!h->stepped || !h->d_pairs || !h->stat_dev) return PLIFE_OK;
+    const int nc = 3 + (int)h->worlds.size();
+    k_copy_u64<<<ceil_div(nc, kBlock), kBlock, 0, h->stream>>>(h->d_pairs, h->stat_dev, nc);
+    LAUNCH_CHECK();
+    return PLIFE_OK;
+}
+
 // Steps plife_update keeps queued on the device.  The reference's stop request
 // takes effect before the next particle (Physics.java:118-133, 149-151); a
 // device step cannot stop halfway, so plife_update queues only about
@@ -1400,18 +1415,24 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
             h->gkey = key;
         }
         while (nsteps - k >= 2) {
-            if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+            if (honour_stop && h->stop.load()) {
+                if (k > 0) (void)copy_stats(h);
+                return PLIFE_STOPPED;
+            }
             HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
             k += 2;
             if ((rc = launched(2))) return rc;
         }
     }
     for (; k < nsteps; ++k) {
-        if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+        if (honour_stop && h->stop.load()) {
+            if (k > 0) (void)copy_stats(h);
+            return PLIFE_STOPPED;
+        }
         if ((rc = do_step(h, nullptr))) return rc;
         if ((rc = launched(1))) return rc;
     }
-    return PLIFE_OK;
+    return k > 0 ? copy_stats(h) : PLIFE_OK;
 }
 
 int time_steps(plife_physics *h, int nsteps, double *ms)
@@ -1462,6 +1483,7 @@ int profile(plife_physics *h, int nsteps, plife_timing *out)
         rc = do_step(h, &ev[k]);
         if (!rc) count_steps(h, 1);
     }
+    if (!rc && nsteps > 0) rc = copy_stats(h);
     if (!rc) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         double sum[4] = {0, 0, 0, 0};
@@ -1871,6 +1893,8 @@ int create_handle(int32_t device, plife_physics *h)
     if (e != hipSuccess) return fail(PLIFE_ERR_MEMORY, "hipHostMalloc: %s", hipGetErrorString(e));
     std::memset(h->stat_host, 0, 4 * sizeof(uint64_t));
     h->stat_cap = 4;
+    e = hipHostGetDevicePointer((void **)&h->stat_dev, h->stat_host, 0);
+    if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
     return PLIFE_OK;
 }
 
@@ -2148,9 +2172,9 @@ int noalloc_step(const plife_settings *s, int32_t ntypes, const double *matrix, 
 #define PLIFE_NA_BIN(DD)                                                                          \
     do {                                                                                          \
         if (d3) k_bin_tile<3, DD><<<plan.ntiles, 1024, 0, stream>>>(px, py, pz, n, t.d_worlds,   \
-                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull);       \
+                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull, nullptr); \
         else k_bin_tile<2, DD><<<plan.ntiles, 1024, 0, stream>>>(px, py, pz, n, t.d_worlds,      \
-                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull);       \
+                nullptr, plan.ntiles, t.keys0, t.cell_count, t.hist, t.d_pairs, 3, 1ull, nullptr); \
     } while (0)
         switch (plan.D) {
             case 1: PLIFE_NA_BIN(1); break;

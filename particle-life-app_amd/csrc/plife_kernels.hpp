@@ -99,6 +99,15 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
     if (i < n) p[i] = v;
 }
 
+// The last step's counters to the host's pinned copy (the step after copies
+// them itself, in k_bin_tile).
+__global__ void k_copy_u64(const unsigned long long *__restrict__ src, unsigned long long *__restrict__ dst,
+                           int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,15 +198,20 @@ __global__ __launch_bounds__(1024) void k_bin_tile(const uint32_t *__restrict__ 
                                                   uint32_t *__restrict__ cell_count,
                                                   uint32_t *__restrict__ hist,
                                                   unsigned long long *__restrict__ counters,
-                                                  int ncounters, unsigned long long tag)
+                                                  int ncounters, unsigned long long tag,
+                                                  unsigned long long *__restrict__ stat_out)
 {
     constexpr int R = 1 << D;
     __shared__ uint32_t h[R];
     for (int t = threadIdx.x; t < R; t += 1024) h[t] = 0;
-    // the step's counters (pairs, sum of squares, per-world pairs) start at zero
+    // the step's counters (pairs, sum of squares, per-world pairs) start at
+    // zero, after the previous step's final values (and its layout tag) have
+    // gone to the host's pinned copy (stat_out, when given)
     if (blockIdx.x == 0) {
-        for (int t = threadIdx.x; t < ncounters; t += 1024) counters[t] = 0ull;
-        if (threadIdx.x == 0) counters[ncounters] = tag;    // which layout the counts belong to
+        for (int t = threadIdx.x; t <= ncounters; t += 1024) {
+            if (stat_out) stat_out[t] = counters[t];
+            counters[t] = t < ncounters ? 0ull : tag;   // [ncounters]: which layout the counts belong to
+        }
     }
     const double cs0 = worlds[0].cs;
     const int nx0 = worlds[0].nx;
