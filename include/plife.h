@@ -206,7 +206,12 @@ int plife_get_accelerations(plife_physics *h, int32_t n, float *acc);
  * for staging to pay, 1 none, 2 all; windowed_only 1: the staged force
  * kernel always screens homes in windows with per-thread candidate lists,
  * never on its balanced path (one window, homes dealt to lanes by their
- * kept-candidate counts).  Results do not depend on the tuning.  Automatic
+ * kept-candidate counts); small_step: whether worlds that fit one
+ * workgroup's LDS (up to about 10,000 particles in 2D, 7,000 in 3D) step in
+ * one launch that bins, sorts and steps them in LDS -- 0 when every world of
+ * the handle fits and no other launch-shape field is set, 1 never, 2
+ * whenever every world fits (block_homes then sets the homes each of its
+ * workgroups steps).  Results do not depend on the tuning.  Automatic
  * fields follow the world's clustering: every step measures the mean size of
  * the cell a particle sits in, and each plife_update plans its launches from
  * the latest measurement that has reached the host. */
@@ -217,6 +222,7 @@ typedef struct {
     double stage_slack;
     int32_t sparse_worlds;
     int32_t windowed_only;
+    int32_t small_step;
 } plife_tuning;
 
 int plife_set_tuning(plife_physics *h, const plife_tuning *t);

@@ -32,6 +32,7 @@
 
 #include "../../include/plife.h"
 #include "plife_force.hpp"
+#include "plife_small.hpp"
 
 using namespace plife;
 
@@ -289,6 +290,19 @@ struct plife_physics {
     size_t last_lds = 0;
     int last_cap = 0, last_bs = 0, last_lcap = 0;
 
+    // the one-launch step of worlds that fit one workgroup's LDS (k_step_small)
+    int lds_limit = 65536;             // the device's LDS per workgroup
+    bool small_ok = false;             // every world fits and the tuning allows the one-launch step
+    bool small = false;                // the current configuration steps in one launch
+    bool small_pref = false;           // the measured choice for layout version small_meas
+    uint64_t small_meas = ~0ull;
+    hipEvent_t tune_ev[2] = {};        // timing of the measured choice
+    int small_homes = 256, small_nmax = 0, small_cmax = 0;
+    size_t small_lds = 0;
+    int small_kbuf = 0;                // which of skeys (0) and kA (1) holds the last step's sorted keys
+    unsigned long long *d_facc = nullptr;   // its counter accumulator (+ ticket), zero between steps
+    size_t facc_cap = 0;
+
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
 };
@@ -537,6 +551,10 @@ int upload_params(plife_physics *h, size_t extra)
     if ((rc = ensure_dev(&h->d_pairs, &h->pairs_cap, (size_t)(3 + W), &h->retired))) return rc;
     if (h->d_pairs != old_pairs)        // per-world counters read before the next step
         HIP_TRY(hipMemsetAsync(h->d_pairs, 0, h->pairs_cap * 8, h->stream));
+    unsigned long long *const old_facc = h->d_facc;
+    if ((rc = ensure_dev(&h->d_facc, &h->facc_cap, (size_t)(3 + W), &h->retired))) return rc;
+    if (h->d_facc != old_facc)          // the one-launch step keeps it zero between steps
+        HIP_TRY(hipMemsetAsync(h->d_facc, 0, h->facc_cap * 8, h->stream));
     if ((size_t)(3 + W) > h->stat_cap) {
         // the old buffer may still be written by queued steps: keep it until destroy
         const size_t cap = grown(h->stat_cap, (size_t)(3 + W));
@@ -1121,6 +1139,111 @@ void classify_worlds(plife_physics *h)
     if (changed) ++h->classes;
 }
 
+// The one-launch step's kernel for the handle's dimension, debug mode and
+// matrix placement.
+template <int DIM, bool DEBUG, bool MLDS>
+const void *small_kernel_of()
+{
+    void (*f)(SmallArgs) = k_step_small<DIM, DEBUG, MLDS>;
+    return reinterpret_cast<const void *>(f);
+}
+
+const void *small_kernel(int dim, bool dbg, bool mlds)
+{
+    if (dim == 3) {
+        if (dbg) return mlds ? small_kernel_of<3, true, true>() : small_kernel_of<3, true, false>();
+        return mlds ? small_kernel_of<3, false, true>() : small_kernel_of<3, false, false>();
+    }
+    if (dbg) return mlds ? small_kernel_of<2, true, true>() : small_kernel_of<2, true, false>();
+    return mlds ? small_kernel_of<2, false, true>() : small_kernel_of<2, false, false>();
+}
+
+// Whether the handle steps in one launch (k_step_small): every world fits one
+// workgroup's LDS (particles, cells, types), and the tuning allows it.  Homes
+// per workgroup: 256 while the workgroups of all worlds fit one pass over the
+// CUs (each workgroup sorts its whole world, so fewer, larger workgroups waste
+// less once the device is full), else 512 or 1024.
+bool plan_small(plife_physics *h)
+{
+    const plife_tuning &t = h->tune;
+    if (t.small_step == 1 || h->n <= 0 || h->worlds.empty()) return false;
+    if (t.small_step == 0 && (t.block_homes || t.list_cap || t.stage_slack > 0.0 || t.sparse_worlds ||
+                              t.windowed_only))
+        return false;
+    const int dim = h->dim == 3 ? 3 : 2;
+    int nmax = 0;
+    long long cmax = 0;
+    for (const WorldHost &w : h->worlds) {
+        const long long nc = grid_of(w.st).ncells;
+        if (w.n > kSmallMaxN || nc > kSmallMaxCells) return false;
+        nmax = std::max(nmax, w.n);
+        cmax = std::max(cmax, nc);
+    }
+    const bool mlds = h->ntypes_max <= kMaxTypesLds;
+    const size_t lds = small_lds_layout(dim, mlds ? h->ntypes_max : 0, nmax, (int)cmax).total;
+    if (lds + 1024 > (size_t)h->lds_limit) return false;     // + the kernel's static LDS
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(small_kernel(dim, h->debug != 0, mlds),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+    }
+    int homes = 256;
+    if (t.small_step == 2 && t.block_homes) {
+        homes = t.block_homes;
+    } else {
+        int cus = 256;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess)
+            cus = 256;
+        for (homes = 256; homes < kSmallThreads; homes *= 2) {
+            long long blocks = 0;
+            for (const WorldHost &w : h->worlds) blocks += std::max(1, ceil_div(w.n, homes));
+            if (blocks <= cus) break;
+        }
+    }
+    h->small_homes = homes;
+    h->small_nmax = nmax;
+    h->small_cmax = (int)cmax;
+    h->small_lds = lds;
+    return true;
+}
+
+// The one-launch step's workgroup table: (world, first home slot) for each
+// run of small_homes homes of every world (one entry for an empty world, which
+// still writes its cell table).
+int ensure_small_blocks(plife_physics *h)
+{
+    if (h->blocks_bs == -h->small_homes && h->blocks_version == h->version &&
+        h->blocks_classes == h->classes)
+        return PLIFE_OK;
+    h->blocks.clear();
+    h->block_key.clear();
+    h->block_est.clear();
+    for (int w = 0; w < (int)h->worlds.size(); ++w) {
+        const WorldHost &wh = h->worlds[w];
+        for (int g = 0; g == 0 || g < wh.n; g += h->small_homes) {
+            h->blocks.push_back(make_uint2((uint32_t)w, (uint32_t)(wh.off + g)));
+            h->block_key.push_back(~0ull);
+            h->block_est.push_back(0u);
+        }
+    }
+    for (int &sg : h->seg) sg = 0;
+    h->nb_total = (int)h->blocks.size();
+    int rc;
+    if ((rc = ensure_dev(&h->d_blocks, &h->blocks_cap, h->blocks.size(), &h->retired))) return rc;
+    ++h->blocks_serial;
+    const size_t bytes = h->blocks.size() * sizeof(uint2);
+    if ((rc = begin_uploads(h, bytes))) return rc;
+    if ((rc = upload(h, h->d_blocks, h->blocks.data(), bytes))) return rc;
+    if ((rc = end_uploads(h))) return rc;
+    h->blocks_bs = -h->small_homes;
+    h->blocks_version = h->version;
+    h->blocks_classes = h->classes;
+    return PLIFE_OK;
+}
+
 // Everything a step of the current configuration needs, so that the steps
 // (captured or direct) upload and allocate nothing.
 int prepare_step(plife_physics *h)
@@ -1139,6 +1262,11 @@ int prepare_step(plife_physics *h)
     h->plan = plan_force(h);
     if ((rc = ensure_cells(h, h->ncells))) return rc;
     if ((rc = ensure_debug(h))) return rc;
+    // the one-launch step where it fits: forced by the tuning, or chosen by
+    // timing both paths on this layout (run_steps); the multi-launch path's
+    // buffers are prepared either way
+    h->small_ok = plan_small(h);
+    h->small = h->small_ok && (h->tune.small_step == 2 || (h->small_meas == h->version && h->small_pref));
     const RadixPlan rp = radix_plan(h->ncells, h->n);
     if ((rc = ensure_dev(&h->hist, &h->hist_cap, rp.hist_len, &h->retired))) return rc;
     const size_t nb = (size_t)ceil_div(h->ncells, kScanTile) +
@@ -1146,13 +1274,102 @@ int prepare_step(plife_physics *h)
     if ((rc = ensure_dev(&h->bsum, &h->bsum_cap, nb, &h->retired))) return rc;
     if ((rc = ensure_dev(&h->bsq, &h->bsq_cap, (size_t)ceil_div(h->ncells, kScanTile), &h->retired)))
         return rc;
+    if (h->small) return ensure_small_blocks(h);
     if ((rc = ensure_blocks(h, h->plan.bs))) return rc;
     return balance_blocks(h);
+}
+
+// Switches the prepared step between the one-launch and the multi-launch path.
+int select_path(plife_physics *h, bool one)
+{
+    h->small = one;
+    if (one) return ensure_small_blocks(h);
+    int rc = ensure_blocks(h, h->plan.bs);
+    return rc ? rc : balance_blocks(h);
+}
+
+// One step of every world in one launch of k_step_small.
+int do_small_step(plife_physics *h, PhaseEvents *ev)
+{
+    int rc;
+    if (h->blocks_bs != -h->small_homes || h->blocks_version != h->version ||
+        h->blocks_classes != h->classes || h->layout_dirty || h->params_dirty)
+        return fail(PLIFE_ERR_STATE, "the step was not prepared");
+    for (int k = 0; k < 4; ++k)
+        if ((rc = record(h, ev, k))) return rc;
+    const int pi = h->pc, ai = h->ac, po = 1 - h->pc, ao = 1 - h->ac;
+    SmallArgs s{};
+    ForceArgs &a = s.f;
+    a.sx = h->px[pi];
+    a.sy = h->py[pi];
+    a.sz = h->pz[pi];
+    a.vx = h->vx[ai];
+    a.vy = h->vy[ai];
+    a.vz = h->vz[ai];
+    a.type = h->type[ai];
+    a.id = h->id[ai];
+    a.ox = h->px[po];
+    a.oy = h->py[po];
+    a.oz = h->pz[po];
+    a.ovx = h->vx[ao];
+    a.ovy = h->vy[ao];
+    a.ovz = h->vz[ao];
+    a.otype = h->type[ao];
+    a.oid = h->id[ao];
+    a.matrix = h->d_matrix;
+    a.worlds = h->d_worlds;
+    a.blocks = h->d_blocks;
+    a.ntypes_max = h->ntypes_max;
+    a.m_in_lds = h->ntypes_max <= kMaxTypesLds ? 1 : 0;
+    a.dbg_count = h->dbg_count;
+    a.dbg_hash = h->dbg_hash;
+    a.dbg_acc = h->dbg_acc;
+    s.perm_out = h->vA;
+    // the last step's sorted keys: the incremental sort's hint (checked in the kernel)
+    s.skeys_in = h->small_kbuf ? h->kA : h->skeys;
+    s.skeys_out = h->small_kbuf ? h->skeys : h->kA;
+    s.keys_out = h->keys0;
+    s.cell_start_out = h->cell_start;
+    s.cell_end_out = h->cell_end;
+    s.acc = h->d_facc;
+    s.counters = h->d_pairs;
+    s.stat_out = h->stat_dev;
+    s.tag = h->version + 1;
+    s.ncounters = 2 + (int)h->worlds.size();
+    s.homes = h->small_homes;
+    s.n_max = h->small_nmax;
+    s.cells_max = h->small_cmax;
+    const bool dbg = h->debug != 0;
+    const int nb = h->nb_total;
+    const size_t lds = h->small_lds;
+    if (h->dim == 3) {
+        if (dbg) a.m_in_lds ? k_step_small<3, true, true><<<nb, kSmallThreads, lds, h->stream>>>(s)
+                            : k_step_small<3, true, false><<<nb, kSmallThreads, lds, h->stream>>>(s);
+        else a.m_in_lds ? k_step_small<3, false, true><<<nb, kSmallThreads, lds, h->stream>>>(s)
+                        : k_step_small<3, false, false><<<nb, kSmallThreads, lds, h->stream>>>(s);
+    } else {
+        if (dbg) a.m_in_lds ? k_step_small<2, true, true><<<nb, kSmallThreads, lds, h->stream>>>(s)
+                            : k_step_small<2, true, false><<<nb, kSmallThreads, lds, h->stream>>>(s);
+        else a.m_in_lds ? k_step_small<2, false, true><<<nb, kSmallThreads, lds, h->stream>>>(s)
+                        : k_step_small<2, false, false><<<nb, kSmallThreads, lds, h->stream>>>(s);
+    }
+    LAUNCH_CHECK();
+    h->stepped = true;
+    h->last_ncells = h->ncells;
+    h->perm = h->vA;
+    h->small_kbuf ^= 1;
+    h->last_lds = lds;
+    h->last_bs = h->small_homes;
+    if ((rc = record(h, ev, 4))) return rc;
+    h->pc = po;
+    h->ac = ao;
+    return PLIFE_OK;
 }
 
 // One step of every world.  prepare_step must have run since the last change.
 int do_step(plife_physics *h, PhaseEvents *ev)
 {
+    if (h->small) return do_small_step(h, ev);
     const int n = h->n;
     const int nc = (int)h->ncells;
     int rc;
@@ -1251,6 +1468,7 @@ int do_step(plife_physics *h, PhaseEvents *ev)
     h->last_bs = fp.bs;
     h->last_lcap = fp.lcap;
     if ((rc = launch_force_any(h, a, fp.bs, fp.lds))) return rc;
+    h->small_kbuf = 0;                 // the last radix pass wrote the sorted keys to skeys
     if ((rc = record(h, ev, 4))) return rc;
     h->pc = po;
     h->ac = ao;
@@ -1266,6 +1484,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
         plife_tuning tune;
         ForcePlan plan;
         int seg[4];
+        int small, small_homes, small_kbuf, pad2;
         uint64_t blocks_serial;
         const void *ptr[40];
     } k;
@@ -1279,6 +1498,9 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
     k.version = h->version;
     k.tune = h->tune;
     k.plan = h->plan;
+    k.small = h->small ? 1 : 0;
+    k.small_homes = h->small_homes;
+    k.small_kbuf = h->small_kbuf;
     std::memcpy(k.seg, h->seg, sizeof k.seg);
     k.blocks_serial = h->blocks_serial;
     const void *p[] = {h->px[0], h->px[1], h->py[0], h->py[1], h->pz[0], h->pz[1], h->vx[0],
@@ -1286,7 +1508,7 @@ std::vector<unsigned char> graph_key(const plife_physics *h)
                        h->id[0], h->id[1], h->keys0, h->kA, h->kB, h->vA, h->vB, h->cell_count,
                        h->cell_start, h->cell_end, h->hist, h->bsum, h->bsq, h->dbg_count, h->dbg_hash, h->dbg_acc,
                        h->d_matrix, h->d_pairs, h->d_worlds, h->world_of, h->d_blocks,
-                       h->stat_host, h->work_host};
+                       h->stat_host, h->work_host, h->skeys, h->d_facc};
     std::memcpy(k.ptr, p, sizeof p);
     std::vector<unsigned char> v(sizeof k);
     std::memcpy(v.data(), &k, sizeof k);
@@ -1400,11 +1622,47 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         count_steps(h, steps);
         return bounded ? throttle(h, steps, limit) : PLIFE_OK;
     };
-    bool use_graph = nsteps >= 2 && !h->tune.direct_launch;
+    // Automatic tuning of a layout that fits the one-launch step: its first
+    // update of at least 6 steps times two steps on each path (after one
+    // warm-up step each; both paths give the same bits, so these are steps
+    // of the update) and keeps the faster path for the layout.  The redundant
+    // per-workgroup sort makes the one-launch step faster for small worlds
+    // and batches that leave CUs idle, slower once the device is full.
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= 6) {
+        float t[2] = {0.0f, 0.0f};
+        for (int p = 0; p < 2; ++p) {
+            if (honour_stop && h->stop.load()) {
+                if (k > 0) (void)copy_stats(h);
+                return PLIFE_STOPPED;
+            }
+            if ((rc = select_path(h, p == 1))) return rc;
+            if ((rc = do_step(h, nullptr))) return rc;
+            HIP_TRY(hipEventRecord(h->tune_ev[0], h->stream));
+            for (int i = 0; i < 2; ++i)
+                if ((rc = do_step(h, nullptr))) return rc;
+            HIP_TRY(hipEventRecord(h->tune_ev[1], h->stream));
+            HIP_TRY(hipEventSynchronize(h->tune_ev[1]));
+            HIP_TRY(hipEventElapsedTime(&t[p], h->tune_ev[0], h->tune_ev[1]));
+            k += 3;
+            if ((rc = launched(3))) return rc;
+        }
+        h->small_pref = t[1] < t[0];
+        h->small_meas = h->version;
+        if ((rc = select_path(h, h->small_pref))) return rc;
+    }
+    bool use_graph = nsteps - k >= 2 && !h->tune.direct_launch;
+    // the one-launch step alternates two key buffers; its graphs start from
+    // buffer 0, after one direct step when the last update ended on buffer 1
+    if (use_graph && h->small && h->small_kbuf != 0 && nsteps - k >= 3) {
+        if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+        if ((rc = do_step(h, nullptr))) return rc;
+        ++k;
+        if ((rc = launched(1))) return rc;
+    }
     if (use_graph) {
         std::vector<unsigned char> key = graph_key(h);
         if (!h->gexec || key != h->gkey) {
-            if (key != h->seen_key && nsteps < 64) use_graph = false;
+            if (key != h->seen_key && nsteps - k < 64) use_graph = false;
             h->seen_key = key;
         }
     }
@@ -1444,6 +1702,11 @@ int time_steps(plife_physics *h, int nsteps, double *ms)
     // build (or reuse) the step graph outside the timed region: the first
     // update of a new configuration launches directly, the second captures
     if ((rc = prepare_step(h))) return rc;
+    // the automatic path choice of this layout, also outside the timed region
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= 6) {
+        if ((rc = run_steps(h, 6, false))) return rc;
+        if ((rc = prepare_step(h))) return rc;
+    }
     for (int i = 0; i < 2 && nsteps >= 2 && !h->tune.direct_launch &&
                     (!h->gexec || graph_key(h) != h->gkey); ++i)
         if ((rc = run_steps(h, 2, false))) return rc;
@@ -1875,6 +2138,10 @@ int create_handle(int32_t device, plife_physics *h)
     h->device = device;
     int rc = set_device(h);
     if (rc) return rc;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        lds > 0)
+        h->lds_limit = lds;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     for (UploadSlot &s : h->up) {
@@ -1887,6 +2154,10 @@ int create_handle(int32_t device, plife_physics *h)
     }
     for (hipEvent_t &ev : h->fl_ev) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    for (hipEvent_t &ev : h->tune_ev) {
+        e = hipEventCreate(&ev);
         if (e != hipSuccess) return fail(PLIFE_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e));
     }
     e = hipHostMalloc((void **)&h->stat_host, 4 * sizeof(uint64_t), hipHostMallocDefault);
@@ -1911,6 +2182,7 @@ void destroy_handle(plife_physics *h)
     dev_free(h->bsum);
     dev_free(h->bsq);
     dev_free(h->d_pairs);
+    dev_free(h->d_facc);
     dev_free(h->d_matrix);
     dev_free(h->d_worlds);
     dev_free(h->world_of);
@@ -1931,6 +2203,10 @@ void destroy_handle(plife_physics *h)
         ev = nullptr;
     }
     for (hipEvent_t &ev : h->fl_ev) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
+    for (hipEvent_t &ev : h->tune_ev) {
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
     }
@@ -1974,7 +2250,7 @@ int check_tuning(const plife_tuning *t)
     if (t->list_cap < 0 || t->list_cap > 512 || !(t->stage_slack >= 0.0) ||
         t->stage_slack > 16.0 || t->direct_launch < 0 || t->direct_launch > 1 ||
         t->sparse_worlds < 0 || t->sparse_worlds > 2 || t->windowed_only < 0 ||
-        t->windowed_only > 1 ||
+        t->windowed_only > 1 || t->small_step < 0 || t->small_step > 2 ||
         (t->block_homes != 0 && t->block_homes != 64 && t->block_homes != 128 &&
          t->block_homes != 256))
         return fail(PLIFE_ERR_ARGUMENT, "tuning out of range");
@@ -1987,6 +2263,7 @@ void report_tuning(const plife_physics *h, plife_tuning *t, int32_t *lds_bytes)
     // automatic fields report what the last force launch used
     if (t->list_cap == 0) t->list_cap = h->last_lcap;
     if (t->block_homes == 0) t->block_homes = h->last_bs;
+    t->small_step = h->small ? 2 : 1;   // the path taken: 2 one launch, 1 multi-launch
     if (lds_bytes) *lds_bytes = (int32_t)h->last_lds;
 }
 
@@ -2508,6 +2785,7 @@ int plife_set_tuning(plife_physics *h, const plife_tuning *t)
     int rc = check_tuning(t);
     if (rc) return rc;
     h->tune = *t;
+    h->small_meas = ~0ull;             // re-time the paths under the new tuning
     return PLIFE_OK;
 }
 

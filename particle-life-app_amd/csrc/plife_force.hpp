@@ -81,29 +81,38 @@ __device__ __forceinline__ unsigned long long id_hash(uint32_t id)
 // With u = |D| = dist R (R = rmax 2^32) and y = 1/|D|: 1/dist = R y, and the
 // attractive branch is (a R) (1 - |1 + beta - 2 S u|/(1 - beta)) y, so the
 // caller passes mR = a * R (staged that way with the matrix).
+template <int DIM>
+__device__ __forceinline__ double pair_d2(double dx, double dy, double dz)
+{
+#pragma clang fp contract(off)
+    // JOML lengthSquared without FMA: x*x + (y*y + z*z) (exact 2^64 scaling)
+    return DIM == 3 ? dx * dx + (dy * dy + dz * dz) : dx * dx + dy * dy;
+}
+
+// f/dist of the force law for a pair in range (0 < d2 <= rmax^2).
+__device__ __forceinline__ double pair_coef(double d2, double mR, const WorldDev &a)
+{
+    const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
+    const double e = d2 * y0;
+    const double h = fma(-e, y0, 1.0);
+    const double y = fma(0.5 * y0, h, y0);        // 1/|D|, one Newton step from FP32
+    const double u = d2 * y;                       // |D| = dist R
+    const double t = fabs(fma(u, a.m2S, a.one_plus_beta));
+    const double g = fma(t, -a.inv_1mb, 1.0);
+    const double att = (mR * g) * y;
+    const double rep = fma(-a.R, y, a.inv_beta);   // 1/beta - 1/dist
+    return u < a.betaR ? rep : att;
+}
+
 template <int DIM, bool DEBUG>
 __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, double mR, uint32_t qid,
                                        const WorldDev &a)
 {
     const double dx = (double)dxi, dy = (double)dyi, dz = (double)dzi;
-    double d2;
-    {
-#pragma clang fp contract(off)
-        // JOML lengthSquared without FMA: x*x + (y*y + z*z) (exact 2^64 scaling)
-        d2 = DIM == 3 ? dx * dx + (dy * dy + dz * dz) : dx * dx + dy * dy;
-    }
+    const double d2 = pair_d2<DIM>(dx, dy, dz);
     const bool in = (d2 != 0.0) & (d2 <= a.r2u);     // Physics.java:432
     if (in) {
-        const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
-        const double e = d2 * y0;
-        const double h = fma(-e, y0, 1.0);
-        const double y = fma(0.5 * y0, h, y0);        // 1/|D|, one Newton step from FP32
-        const double u = d2 * y;                       // |D| = dist R
-        const double t = fabs(fma(u, a.m2S, a.one_plus_beta));
-        const double g = fma(t, -a.inv_1mb, 1.0);
-        const double att = (mR * g) * y;
-        const double rep = fma(-a.R, y, a.inv_beta);   // 1/beta - 1/dist
-        const double c = u < a.betaR ? rep : att;
+        const double c = pair_coef(d2, mR, a);
         acc.x = fma(c, dx, acc.x);
         acc.y = fma(c, dy, acc.y);
         if constexpr (DIM == 3) acc.z = fma(c, dz, acc.z);

@@ -33,14 +33,20 @@ constexpr double kInvFix = 1.0 / 4294967296.0;
 // ---------------------------------------------------------------------------
 // wave64 / block helpers
 // ---------------------------------------------------------------------------
+// Inclusive scan over a wave64 in registers (DPP, no LDS round trips):
+// Hillis-Steele steps of 1, 2, 4, 8 lanes inside each row of 16 (lanes
+// shifted in from outside the row read 0), then row 0's total added to row 1
+// and row 2's to row 3 (row_bcast:15), and the first two rows' total to rows
+// 2 and 3 (row_bcast:31).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, kWave);
-        if (lane >= d) v += t;
-    }
+    constexpr int kRowShr = 0x110, kBcast15 = 0x142, kBcast31 = 0x143;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 1, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 2, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 4, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 8, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kBcast15, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kBcast31, 0xc, 0xf, false);
     return v;
 }
 
@@ -163,6 +169,26 @@ __device__ __forceinline__ int key_axis(uint32_t k, double cs, int nx)
 {
     const double x = (double)k * kInvFix;   // exact
     int c = (int)(x / cs);                   // IEEE FP64 division, x >= 0: trunc == floor
+    return c == nx ? nx - 1 : c;
+}
+
+// trunc(x / cs) for x = k 2^-32 without the division where that is safe:
+// x * (1 / cs) is within 3 ulp of the rounded quotient, i.e. within 2^-35
+// for quotients below 2^16 cells, so its integer part is the division's
+// unless the product lies within 1e-9 of an integer -- then divide.
+__device__ __forceinline__ int trunc_div_fast(uint32_t k, double cs, double ics)
+{
+    const double x = (double)k * kInvFix;
+    const double q = x * ics;
+    int c = (int)q;
+    const double f = q - (double)c;
+    if (f < 1e-9 || f > 1.0 - 1e-9) c = (int)(x / cs);
+    return c;
+}
+
+__device__ __forceinline__ int key_axis_fast(uint32_t k, double cs, double ics, int nx)
+{
+    const int c = trunc_div_fast(k, cs, ics);
     return c == nx ? nx - 1 : c;
 }
 

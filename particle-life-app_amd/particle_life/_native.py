@@ -102,6 +102,7 @@ class PlifeTuning(ctypes.Structure):
         ("stage_slack", ctypes.c_double),
         ("sparse_worlds", ctypes.c_int32),
         ("windowed_only", ctypes.c_int32),
+        ("small_step", ctypes.c_int32),
     ]
 
 

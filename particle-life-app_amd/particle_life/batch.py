@@ -159,9 +159,11 @@ class Batch:
     # tuning and timing
     def set_tuning(self, list_cap: int = 0, stage_slack: float = 0.0,
                    direct_launch: bool = False, block_homes: int = 0,
-                   sparse_worlds: int = 0, windowed_only: bool = False) -> None:
+                   sparse_worlds: int = 0, windowed_only: bool = False,
+                   small_step: int = 0) -> None:
         t = PlifeTuning(int(block_homes), int(list_cap), 1 if direct_launch else 0,
-                        float(stage_slack), int(sparse_worlds), 1 if windowed_only else 0)
+                        float(stage_slack), int(sparse_worlds), 1 if windowed_only else 0,
+                        int(small_step))
         check(self._lib.plife_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -170,7 +172,8 @@ class Batch:
         check(self._lib.plife_batch_get_tuning(self._h, ctypes.byref(t), ctypes.byref(lds)))
         return {"block_homes": t.block_homes, "list_cap": t.list_cap, "stage_slack": t.stage_slack,
                 "direct_launch": bool(t.direct_launch), "sparse_worlds": t.sparse_worlds,
-                "windowed_only": bool(t.windowed_only), "lds_bytes": lds.value}
+                "windowed_only": bool(t.windowed_only), "small_step": t.small_step,
+                "lds_bytes": lds.value}
 
     def time_steps(self, nsteps: int) -> float:
         ms = ctypes.c_double()
