@@ -1160,9 +1160,10 @@ const void *small_kernel(int dim, bool dbg, bool mlds)
 
 // Whether the handle steps in one launch (k_step_small): every world fits one
 // workgroup's LDS (particles, cells, types), and the tuning allows it.  Homes
-// per workgroup: 256 while the workgroups of all worlds fit one pass over the
-// CUs (each workgroup sorts its whole world, so fewer, larger workgroups waste
-// less once the device is full), else 512 or 1024.
+// per workgroup: the fewest of 256, 512, 1024, ... for which the workgroups of
+// all worlds fit one pass over the CUs (each workgroup sorts its whole world,
+// so fewer, larger workgroups waste less once the device is full); when none
+// does, one workgroup per world.
 bool plan_small(plife_physics *h)
 {
     const plife_tuning &t = h->tune;
@@ -1197,11 +1198,12 @@ bool plan_small(plife_physics *h)
         int cus = 256;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess)
             cus = 256;
-        for (homes = 256; homes < kSmallThreads; homes *= 2) {
+        for (homes = 256;; homes *= 2) {
             long long blocks = 0;
             for (const WorldHost &w : h->worlds) blocks += std::max(1, ceil_div(w.n, homes));
-            if (blocks <= cus) break;
+            if (blocks <= cus || homes >= nmax) break;
         }
+        homes = std::max(256, std::min(homes, ((nmax + 255) / 256) * 256));
     }
     h->small_homes = homes;
     h->small_nmax = nmax;

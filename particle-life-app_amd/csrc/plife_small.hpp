@@ -53,7 +53,7 @@ struct SmallArgs {
     unsigned long long *stat_out;    // the host's pinned copy (may be null)
     unsigned long long tag;
     int ncounters;
-    int homes;                       // homes per workgroup
+    int homes;                       // homes per workgroup (any count; stepped kSmallThreads at a time)
     int n_max, cells_max;            // largest world of the launch (LDS layout)
 };
 
@@ -551,22 +551,29 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
         q = block_sum_u64<kSmallThreads>(q, s_q);
     }
 
-    // 5. force and integration of the workgroup's homes
-    const int hl = first + tid;
-    const bool active = tid < s.homes && hl < n;
-    Acc acc;
-    acc.x = acc.y = acc.z = 0.0;
-    acc.cnt = 0;
-    acc.hash = 0ull;
-    if (active) {
-        const uint32_t hx = SX[hl], hy = SY[hl], hz = DIM == 3 ? SZ[hl] : 0u;
-        const int htype = ST[hl];
-        const uint32_t ho = (uint32_t)(off + min((int)(src[hl] & 0xFFFFu), n - 1));
-        small_home<DIM, DEBUG, MLDS>(a, w, M, SX, SY, SZ, ST, src, cst, hl, hx, hy, hz, htype * w.ntypes, acc);
-        if (w.wrap) finish_home<DIM, true, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
-        else finish_home<DIM, false, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+    // 5. force and integration of the workgroup's homes [first, first + homes),
+    //    kSmallThreads at a time (a workgroup may own a whole world of a batch,
+    //    so that the world is sorted once)
+    const int hend = min(first + s.homes, n);
+    unsigned long long mine = 0;
+#pragma unroll 1
+    for (int h0 = first; h0 < hend; h0 += kSmallThreads) {
+        const int hl = h0 + tid;
+        if (hl < hend) {
+            Acc acc;
+            acc.x = acc.y = acc.z = 0.0;
+            acc.cnt = 0;
+            acc.hash = 0ull;
+            const uint32_t hx = SX[hl], hy = SY[hl], hz = DIM == 3 ? SZ[hl] : 0u;
+            const int htype = ST[hl];
+            const uint32_t ho = (uint32_t)(off + min((int)(src[hl] & 0xFFFFu), n - 1));
+            small_home<DIM, DEBUG, MLDS>(a, w, M, SX, SY, SZ, ST, src, cst, hl, hx, hy, hz, htype * w.ntypes, acc);
+            if (w.wrap) finish_home<DIM, true, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+            else finish_home<DIM, false, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+            mine += (unsigned long long)acc.cnt;
+        }
     }
-    const unsigned long long tot = block_sum_u64<kSmallThreads>(active ? (unsigned long long)acc.cnt : 0ull, s_q);
+    const unsigned long long tot = block_sum_u64<kSmallThreads>(mine, s_q);
     // 6. counters.  Agent-scope atomics on one location are coherent across
     //    the XCDs; the ticket's increment depends on the values returned by
     //    this workgroup's adds, so it is issued only after they are performed,
