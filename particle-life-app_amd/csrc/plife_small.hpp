@@ -58,7 +58,7 @@ struct SmallArgs {
 };
 
 struct SmallLds {
-    size_t m, p0, p1, cnt, cst, ocst, mcst, sy, sz, st, total;
+    size_t m, p0, p1, cnt, cst, sy, sz, st, total;
 };
 
 // The sorted entries ping-pong between p0 and p1; after the sort the buffer
@@ -74,8 +74,6 @@ __host__ __device__ inline SmallLds small_lds_layout(int dim, int ntypes_lds, in
     L.p1 = o;   o += (size_t)n_max * 4;
     L.cnt = o;  o += (size_t)kSmallWaves * (1 << kSmallDigitBits) * 4;
     L.cst = o;  o += (size_t)(cells_max + 1) * 2;
-    L.ocst = o; o += (size_t)(cells_max + 1) * 2;
-    L.mcst = o; o += (size_t)(cells_max + 1) * 2;
     o = (o + 3) & ~(size_t)3;
     L.sy = o;   o += (size_t)n_max * 4;
     L.sz = o;   o += dim == 3 ? (size_t)n_max * 4 : 0;
@@ -85,13 +83,12 @@ __host__ __device__ inline SmallLds small_lds_layout(int dim, int ntypes_lds, in
 }
 
 // Candidates lo..hi-1 of the world's sorted order for the home at sorted
-// position hl: visit_run over LDS, kSmallU candidates at a time with no
-// branch inside a group, so that their loads are issued together and their
-// force coefficients run as independent FP64 chains side by side (one wave
-// per SIMD has no other wave to hide their latency behind).  The sums take
-// the candidates in order; a candidate out of range adds the coefficient 0,
-// which leaves a sum bit-identical (a sum of nonzero terms is never -0 in
-// round-to-nearest, and x + (+-0) = x otherwise), as skipping it does.
+// position hl: visit_run over LDS in two phases per 32 candidates.  The FP32
+// screen of k_force (exact int32 deltas, |D|^2 <= rmax^2 (1 + 2^-20), a strict
+// superset of the FP64 predicate) runs kSmallU candidates at a time with their
+// loads issued together and sets one bit per kept candidate; then the kept
+// candidates, in order, take the reference's FP64 predicate, the clamped
+// world's sign test and the force law.  The sums are those of visit_run.
 template <int DIM, bool DEBUG, bool MLDS>
 __device__ __forceinline__ void small_run(const ForceArgs &a, const WorldDev &w, bool wrap, const double *M,
                                           const uint32_t *SX, const uint32_t *SY, const uint32_t *SZ,
@@ -99,48 +96,40 @@ __device__ __forceinline__ void small_run(const ForceArgs &a, const WorldDev &w,
                                           uint32_t hy, uint32_t hz, int row, Acc &acc, int lo, int hi)
 {
 #pragma unroll 1
-    for (int j0 = lo; j0 < hi; j0 += kSmallU) {
-        uint32_t qx[kSmallU], qy[kSmallU], qz[kSmallU];
-        int tp[kSmallU];
+    for (int c0 = lo; c0 < hi; c0 += 32) {
+        const int cn = min(32, hi - c0);
+        uint32_t mask = 0;
+#pragma unroll 1
+        for (int u0 = 0; u0 < cn; u0 += kSmallU) {
+            uint32_t qx[kSmallU], qy[kSmallU], qz[kSmallU];
 #pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
-            const int j = min(j0 + u, hi - 1);
-            qx[u] = SX[j];
-            qy[u] = SY[j];
-            qz[u] = DIM == 3 ? SZ[j] : 0u;
-            tp[u] = ST[j];
+            for (int u = 0; u < kSmallU; ++u) {
+                const int j = min(c0 + u0 + u, hi - 1);
+                qx[u] = SX[j];
+                qy[u] = SY[j];
+                qz[u] = DIM == 3 ? SZ[j] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kSmallU; ++u)
+                if (u0 + u < cn && screen<DIM>(hx, hy, hz, qx[u], qy[u], qz[u], w.r2hi)) mask |= 1u << (u0 + u);
         }
-        double m[kSmallU];
-#pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
-            const int idx = row + tp[u];
-            m[u] = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
-        }
-        double c[kSmallU], dx[kSmallU], dy[kSmallU], dz[kSmallU];
-        bool use[kSmallU];
-#pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
-            const int j = j0 + u;
-            const int dxi = (int)(qx[u] - hx), dyi = (int)(qy[u] - hy);
-            const int dzi = DIM == 3 ? (int)(qz[u] - hz) : 0;
-            // plain difference of a clamped world: skip |D| >= 1/2
-            bool same = ((dxi >= 0) == (qx[u] >= hx)) & ((dyi >= 0) == (qy[u] >= hy));
-            if (DIM == 3) same &= ((dzi >= 0) == (qz[u] >= hz));
-            dx[u] = (double)dxi;
-            dy[u] = (double)dyi;
-            dz[u] = (double)dzi;
-            const double d2 = pair_d2<DIM>(dx[u], dy[u], dz[u]);
-            use[u] = (j < hi) & (j != hl) & (wrap | same) & (d2 != 0.0) & (d2 <= w.r2u);   // Physics.java:432
-            c[u] = pair_coef(use[u] ? d2 : 1.0, m[u], w) * (use[u] ? 1.0 : 0.0);
-        }
-#pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
-            acc.x = fma(c[u], dx[u], acc.x);
-            acc.y = fma(c[u], dy[u], acc.y);
-            if constexpr (DIM == 3) acc.z = fma(c[u], dz[u], acc.z);
-            acc.cnt += use[u] ? 1 : 0;
-            if constexpr (DEBUG)
-                if (use[u]) acc.hash += id_hash((uint32_t)a.id[w.off + min((int)(P[j0 + u] & 0xFFFFu), w.n - 1)]);
+        if (hl >= c0 && hl < c0 + 32) mask &= ~(1u << (hl - c0));
+        while (mask) {
+            const int j = c0 + __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t qx = SX[j], qy = SY[j], qz = DIM == 3 ? SZ[j] : 0u;
+            const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
+            const int dzi = DIM == 3 ? (int)(qz - hz) : 0;
+            if (!wrap) {
+                // plain difference of a clamped world: skip |D| >= 1/2
+                bool ok = ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
+                if (DIM == 3) ok &= ((dzi >= 0) == (qz >= hz));
+                if (!ok) continue;
+            }
+            const int idx = row + ST[j];
+            const double m = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
+            pair64<DIM, DEBUG>(acc, dxi, dyi, dzi, m,
+                               DEBUG ? (uint32_t)a.id[w.off + min((int)(P[j] & 0xFFFFu), w.n - 1)] : 0u, w);
         }
     }
 }
@@ -319,8 +308,6 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
     uint32_t *P1 = reinterpret_cast<uint32_t *>(smem + L.p1);
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + L.cnt);
     uint16_t *cst = reinterpret_cast<uint16_t *>(smem + L.cst);
-    uint16_t *ocst = reinterpret_cast<uint16_t *>(smem + L.ocst);
-    uint16_t *mcst = reinterpret_cast<uint16_t *>(smem + L.mcst);
     uint32_t *SY = reinterpret_cast<uint32_t *>(smem + L.sy);
     uint32_t *SZ = reinterpret_cast<uint32_t *>(smem + L.sz);
     uint8_t *ST = smem + L.st;
@@ -435,38 +422,42 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
                 else MV[i - (int)sp] = e;
             }
             if (tid == 0) Sp[n] = (uint16_t)total;
-            small_table(ocst, n, nc, [&](int j) { return (int)O[j]; });
             __syncthreads();
             uint32_t *mv = MV2;
             if (m <= kSmallThreads) small_rank_sort(MV, MV2, m);
             else mv = small_radix_sort(MV, MV2, m, kbits, cnt, s_w);
             __syncthreads();
-            small_table(mcst, m, nc, [&](int j) { return (int)(mv[j] >> 16); });
-            __syncthreads();
-            // cst: exclusive scan of the new counts over contiguous cell chunks
-            auto count_of = [&](int c) -> uint32_t {
-                return (uint32_t)(Sp[ocst[c + 1]] - Sp[ocst[c]]) + (uint32_t)(mcst[c + 1] - mcst[c]);
-            };
-            const int cc = (nc + kSmallThreads - 1) / kSmallThreads;
-            const int b0 = min(tid * cc, nc), b1 = min(b0 + cc, nc);
-            uint32_t sum = 0;
-#pragma unroll 1
-            for (int r0 = 0; r0 < cc; r0 += kSmallU) {
-                uint32_t v[kSmallU];
+            // A mover of sorted rank q, key c and slot i lands after every
+            // stayer of a smaller key or of key c and a smaller slot -- the
+            // stayers before slot i clamped to O's run of c -- and after the q
+            // movers before it.  Positions first: O shares P1 with the output.
+            int mpos[kSmallPer];
 #pragma unroll
-                for (int u = 0; u < kSmallU; ++u) v[u] = count_of(min(b0 + r0 + u, nc - 1));
-#pragma unroll
-                for (int u = 0; u < kSmallU; ++u) sum += b0 + r0 + u < b1 ? v[u] : 0u;
+            for (int r = 0; r < kSmallPer; ++r) {
+                const int q = tid + r * kSmallThreads;
+                mpos[r] = 0;
+                if (q < m) {
+                    const uint32_t ee = mv[q];
+                    const int c = (int)(ee >> 16), i = (int)(ee & 0xFFFFu);
+                    int lo = 0, hi = n;              // first slot with O >= c
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((int)O[mid] < c) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    int lo2 = lo, hi2 = n;           // first slot with O > c
+                    while (lo2 < hi2) {
+                        const int mid = (lo2 + hi2) >> 1;
+                        if ((int)O[mid] <= c) lo2 = mid + 1;
+                        else hi2 = mid;
+                    }
+                    mpos[r] = (int)Sp[min(max(i, lo), lo2)] + q;
+                }
             }
-            uint32_t base = block_excl_scan<kSmallThreads>(sum, s_w, total);
-#pragma unroll 1
-            for (int c = b0; c < b1; ++c) {
-                cst[c] = (uint16_t)base;
-                base += count_of(c);
-            }
-            if (tid == 0) cst[nc] = (uint16_t)n;
             __syncthreads();
-            // placement into P1 (O is no longer read: stayers are Sp steps)
+            // A stayer of key c at slot i lands after the stayers before it
+            // (all of a smaller key, or of key c and a smaller slot) and the
+            // movers of a smaller key, or of key c and a smaller slot.
 #pragma unroll 1
             for (int g = 0; g < ngroups; ++g) {
 #pragma unroll
@@ -474,22 +465,23 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
                     const int i = tid + (g * kSmallU + u) * kSmallThreads;
                     const int ic = min(i, n - 1);
                     const uint32_t ee = P0[ic];
-                    const int c = (int)(ee >> 16);
-                    const int s0 = Sp[ic], s1 = Sp[ic + 1], so = Sp[ocst[c]];
-                    const int q0 = mcst[c], q1 = mcst[c + 1], at = cst[c];
-                    if (i < n && s1 != s0) {
-                        int before = s0 - so;
-                        for (int q = q0; q < q1 && (int)(mv[q] & 0xFFFFu) < i; ++q) ++before;
-                        P1[at + before] = ee;
+                    if (i < n && Sp[ic + 1] != Sp[ic]) {
+                        const int c = (int)(ee >> 16);
+                        int lo = 0, hi = m;          // movers of a smaller key
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if ((int)(mv[mid] >> 16) < c) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        while (lo < m && mv[lo] < ((uint32_t)c << 16 | (uint32_t)i)) ++lo;
+                        P1[(int)Sp[ic] + lo] = ee;
                     }
                 }
             }
-            for (int q = tid; q < m; q += kSmallThreads) {
-                const uint32_t ee = mv[q];
-                const int c = (int)(ee >> 16), i = (int)(ee & 0xFFFFu);
-                const int lo = ocst[c], hi = ocst[c + 1];
-                const int at = min(max(i, lo), hi);
-                P1[cst[c] + ((int)Sp[at] - (int)Sp[lo]) + (q - (int)mcst[c])] = ee;
+#pragma unroll
+            for (int r = 0; r < kSmallPer; ++r) {
+                const int q = tid + r * kSmallThreads;
+                if (q < m) P1[mpos[r]] = mv[q];
             }
             __syncthreads();
             src = P1;
