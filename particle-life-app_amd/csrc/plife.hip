@@ -1624,6 +1624,20 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         count_steps(h, steps);
         return bounded ? throttle(h, steps, limit) : PLIFE_OK;
     };
+    // The first update of a 3D handle: the launch plan follows the clustering
+    // of the cells, which only a step measures, so a long first update steps
+    // once, waits for that step's counters and re-plans the rest (C3: the
+    // clustered plan is 1.6x faster than the uniform one it would otherwise run).
+    if (h->dim == 3 && h->occupancy <= 0.0 && !h->small && nsteps - k >= 3 && h->n > 0 &&
+        h->tune.block_homes == 0) {
+        if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+        if ((rc = do_step(h, nullptr))) return rc;
+        ++k;
+        if ((rc = launched(1))) return rc;
+        if ((rc = copy_stats(h))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if ((rc = prepare_step(h))) return rc;
+    }
     // Automatic tuning of a layout that fits the one-launch step: its first
     // update of at least 6 steps times two steps on each path (after one
     // warm-up step each; both paths give the same bits, so these are steps

@@ -260,6 +260,29 @@ def test_planner_detects_clustering(eng):
     assert eng.tuning()["block_homes"] == 256
 
 
+def test_first_update_replans_after_one_step(eng):
+    """A clustered 3D world's first long update runs its first step on the
+    uniform plan, measures the clustering and steps the rest on the clustered
+    plan, with the bits of a world stepped one update at a time."""
+    from particle_life import Engine, worlds
+
+    print("\n[parity] case=first_update_replans", flush=True)
+    eng.set_tuning()
+    w = worlds.gaussian_blob_world(128, 120_000, 12, rmax=0.01, dt=0.01, beta=0.25, nblobs=8,
+                                   sigma=0.008, dim=3)
+    eng.load_world(w)
+    eng.step(4)
+    assert eng.tuning()["block_homes"] == 128
+    got = eng.state()
+    with Engine(0) as e:
+        e.load_world(w)
+        for _ in range(4):
+            e.step(1)
+        ref = e.state()
+    for k in ("pos", "vel", "ids", "types"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
 def test_edge_sizes(eng):
     """Empty world, one particle, two particles, coincident particles."""
     from particle_life import worlds
