@@ -10,18 +10,20 @@
 // launches: a world of n particles takes ceil(n / homes) workgroups of 1024
 // threads.  Each workgroup bins and sorts the whole world itself (a few
 // microseconds of one CU; sharing the sort would need a grid-wide barrier)
-// and steps `homes` consecutive particles of the sorted order.  Every home
+// and steps `homes` consecutive particles of the sorted order, 1024 at a
+// time; with many worlds a workgroup owns a whole world.  The sort is
+// incremental where it can be: the slots are the previous step's sorted
+// order, so only the particles that changed cell are ranked.  Every home
 // visits its candidates in the order of the other force kernels (home_global:
 // per stencil row, cells left to right, each cell in sorted order) with the
-// same FP64 force law and integrator, so the new state is bit-identical to
-// the multi-launch path's.
+// same FP32 superset screen, FP64 force law and integrator, so the new state
+// is bit-identical to the multi-launch path's.
 //
-// The step is short and runs its code once per workgroup, so code size is
-// kept small (instruction fetch, not arithmetic, bounds a step of ten
-// thousand particles when the code outgrows the instruction cache): the
-// boundary mode is a run-time flag, loops are unrolled four deep, with
-// clamped indices so that each group of four loads is issued together, and
-// the candidate loop reads the next candidate while it evaluates the current.
+// A step is a chain of short phases on one CU, so per-thread loops are
+// unrolled four deep with clamped indices: each group of four loads is
+// issued together instead of one round trip per iteration.  Loops are not
+// unrolled further, and the boundary mode is a run-time flag, to keep the
+// code (run once per workgroup) small.
 //
 // Counters: workgroups add their pairs into an accumulator; the last
 // workgroup to finish (a ticket counter, no waiting) publishes the step's
