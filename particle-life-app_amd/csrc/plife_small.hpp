@@ -10,7 +10,9 @@
 // launches: a world of n particles takes ceil(n / homes) workgroups of 1024
 // threads.  Each workgroup bins and sorts the whole world itself (a few
 // microseconds of one CU; sharing the sort would need a grid-wide barrier)
-// and steps `homes` consecutive particles of the sorted order, 1024 at a
+// and steps `homes` consecutive particles of the sorted order: with at most
+// 256 homes, 4 or 16 lanes share each home (one lane's chain of dependent
+// loads and FP64 steps is the step's critical path), else 1024 homes at a
 // time; with many worlds a workgroup owns a whole world.  The sort is
 // incremental where it can be: the slots are the previous step's sorted
 // order, so only the particles that changed cell are ranked.  Every home
@@ -55,7 +57,7 @@ struct SmallArgs {
     unsigned long long *stat_out;    // the host's pinned copy (may be null)
     unsigned long long tag;
     int ncounters;
-    int homes;                       // homes per workgroup (any count; stepped kSmallThreads at a time)
+    int homes;                       // homes per workgroup (any count; <= 64: 16 lanes per home, <= 256: 4)
     int n_max, cells_max;            // largest world of the launch (LDS layout)
 };
 
@@ -136,15 +138,83 @@ __device__ __forceinline__ void small_run(const ForceArgs &a, const WorldDev &w,
     }
 }
 
+// The same candidates with G lanes per home (G = 4: a quad of lanes; G = 16:
+// a row of 16), for workgroups with fewer homes than threads: lane sub of the
+// home's group evaluates candidate j0 + sub of each step of G candidates --
+// the reference's FP64 predicate, the clamped world's sign test and the force
+// law, with no screen and no branch (c = 0 where the pair does not interact)
+// -- and then every lane of the group adds the step's G terms in candidate
+// order, acc = fma(c_k, D_k, acc) for k = 0 .. G-1, with each lane's c and D
+// broadcast across the group by DPP.  These are small_run's operations in its
+// order (a term with c = 0 leaves a sum unchanged: a sum that starts at +0 is
+// never -0), so the sums are bit-identical, while one home's chain of
+// dependent loads and FP64 steps is shared by G lanes.  Interacting pairs
+// (and the debug hash) are counted per lane; the caller sums the group.
+template <int G, int K>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v)
+{
+    static_assert(G == 4 || G == 16, "groups are DPP quads or rows");
+    if constexpr (G == 4) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, K * 0x55, 0xf, 0xf, false);  // quad_perm:[K,K,K,K]
+    else return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x150 + K, 0xf, 0xf, false);  // row_newbcast:K
+}
+
+template <int DIM, int G, int K = 0>
+__device__ __forceinline__ void group_terms(Acc &acc, double c, int dx, int dy, int dz)
+{
+    if constexpr (K < G) {
+        const unsigned long long cb = (unsigned long long)__double_as_longlong(c);
+        const unsigned long long kb = (unsigned long long)group_bcast<G, K>((uint32_t)cb) |
+                                      ((unsigned long long)group_bcast<G, K>((uint32_t)(cb >> 32)) << 32);
+        const double ck = __longlong_as_double((long long)kb);
+        acc.x = fma(ck, (double)(int)group_bcast<G, K>((uint32_t)dx), acc.x);
+        acc.y = fma(ck, (double)(int)group_bcast<G, K>((uint32_t)dy), acc.y);
+        if constexpr (DIM == 3) acc.z = fma(ck, (double)(int)group_bcast<G, K>((uint32_t)dz), acc.z);
+        group_terms<DIM, G, K + 1>(acc, c, dx, dy, dz);
+    }
+}
+
+template <int DIM, bool DEBUG, bool MLDS, int G>
+__device__ __forceinline__ void small_run_split(const ForceArgs &a, const WorldDev &w, bool wrap,
+                                                const double *M, const uint32_t *SX, const uint32_t *SY,
+                                                const uint32_t *SZ, const uint8_t *ST, const uint32_t *P,
+                                                int sub, uint32_t hx, uint32_t hy, uint32_t hz, int row,
+                                                Acc &acc, int lo, int hi)
+{
+#pragma unroll 1
+    for (int j0 = lo; j0 < hi; j0 += G) {
+        const int j = j0 + sub;
+        const int jc = min(j, hi - 1);
+        const uint32_t qx = SX[jc], qy = SY[jc], qz = DIM == 3 ? SZ[jc] : 0u;
+        const int idx = row + ST[jc];
+        const double m = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
+        const int dxi = (int)(qx - hx), dyi = (int)(qy - hy);
+        const int dzi = DIM == 3 ? (int)(qz - hz) : 0;
+        bool in = j < hi;
+        if (!wrap) {
+            // plain difference of a clamped world: skip |D| >= 1/2
+            in &= ((dxi >= 0) == (qx >= hx)) & ((dyi >= 0) == (qy >= hy));
+            if (DIM == 3) in &= ((dzi >= 0) == (qz >= hz));
+        }
+        const double d2 = pair_d2<DIM>((double)dxi, (double)dyi, (double)dzi);
+        in &= (d2 != 0.0) & (d2 <= w.r2u);           // Physics.java:432
+        const double c = in ? pair_coef(d2, m, w) : 0.0;
+        acc.cnt += in ? 1 : 0;
+        if constexpr (DEBUG)
+            if (in) acc.hash += id_hash(a.id[w.off + min((int)(P[jc] & 0xFFFFu), w.n - 1)]);
+        group_terms<DIM, G>(acc, c, dxi, dyi, dzi);
+    }
+}
+
 // home_global's stencil (Physics.java:404-424) over the LDS cell table cst
 // (cell c holds sorted positions cst[c] .. cst[c + 1] - 1).  Each visited
-// run of cells goes through one call site of small_run.
-template <int DIM, bool DEBUG, bool MLDS>
+// run of cells goes through one call site of small_run (G = 1: one lane per
+// home) or small_run_split (G lanes per home; lane sub of the group).
+template <int DIM, bool DEBUG, bool MLDS, int G>
 __device__ __forceinline__ void small_home(const ForceArgs &a, const WorldDev &w, const double *M,
                                            const uint32_t *SX, const uint32_t *SY, const uint32_t *SZ,
                                            const uint8_t *ST, const uint32_t *P, const uint16_t *cst,
-                                           int hl, uint32_t hx, uint32_t hy, uint32_t hz, int row,
-                                           Acc &acc)
+                                           int hl, int sub, uint32_t hx, uint32_t hy, uint32_t hz,
+                                           int row, Acc &acc)
 {
     const int nx = w.nx;
     const bool wrap = w.wrap != 0;
@@ -183,7 +253,11 @@ __device__ __forceinline__ void small_home(const ForceArgs &a, const WorldDev &w
             lo = cst[ci];
             hi = cst[ci + 1];
         }
-        small_run<DIM, DEBUG, MLDS>(a, w, wrap, M, SX, SY, SZ, ST, P, hl, hx, hy, hz, row, acc, lo, hi);
+        if constexpr (G == 1)
+            small_run<DIM, DEBUG, MLDS>(a, w, wrap, M, SX, SY, SZ, ST, P, hl, hx, hy, hz, row, acc, lo, hi);
+        else
+            small_run_split<DIM, DEBUG, MLDS, G>(a, w, wrap, M, SX, SY, SZ, ST, P, sub, hx, hy, hz, row, acc,
+                                                 lo, hi);
     }
 }
 
@@ -545,26 +619,58 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
         q = block_sum_u64<kSmallThreads>(q, s_q);
     }
 
-    // 5. force and integration of the workgroup's homes [first, first + homes),
-    //    kSmallThreads at a time (a workgroup may own a whole world of a batch,
-    //    so that the world is sorted once)
+    // 5. force and integration of the workgroup's homes [first, first + homes):
+    //    G lanes per home where the homes leave threads free (G = 16 for at
+    //    most 64 homes, 4 for at most 256), else kSmallThreads homes at a time
+    //    (a workgroup may own a whole world of a batch, so that the world is
+    //    sorted once)
     const int hend = min(first + s.homes, n);
     unsigned long long mine = 0;
-#pragma unroll 1
-    for (int h0 = first; h0 < hend; h0 += kSmallThreads) {
-        const int hl = h0 + tid;
-        if (hl < hend) {
+    const int gsh = s.homes <= kSmallThreads / 16 ? 4 : s.homes <= kSmallThreads / 4 ? 2 : 0;   // log2 G
+    if (gsh > 0) {
+        const int hl = first + (tid >> gsh), sub = tid & ((1 << gsh) - 1);
+        if (hl < hend) {                         // the whole group of the home
             Acc acc;
             acc.x = acc.y = acc.z = 0.0;
             acc.cnt = 0;
             acc.hash = 0ull;
             const uint32_t hx = SX[hl], hy = SY[hl], hz = DIM == 3 ? SZ[hl] : 0u;
             const int htype = ST[hl];
-            const uint32_t ho = (uint32_t)(off + min((int)(src[hl] & 0xFFFFu), n - 1));
-            small_home<DIM, DEBUG, MLDS>(a, w, M, SX, SY, SZ, ST, src, cst, hl, hx, hy, hz, htype * w.ntypes, acc);
-            if (w.wrap) finish_home<DIM, true, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
-            else finish_home<DIM, false, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
-            mine += (unsigned long long)acc.cnt;
+            if (gsh == 4)
+                small_home<DIM, DEBUG, MLDS, 16>(a, w, M, SX, SY, SZ, ST, src, cst, hl, sub, hx, hy, hz,
+                                                 htype * w.ntypes, acc);
+            else
+                small_home<DIM, DEBUG, MLDS, 4>(a, w, M, SX, SY, SZ, ST, src, cst, hl, sub, hx, hy, hz,
+                                                htype * w.ntypes, acc);
+            for (int d = 1; d < (1 << gsh); d <<= 1) {
+                acc.cnt += __shfl_xor(acc.cnt, d, kWave);
+                if constexpr (DEBUG) acc.hash += __shfl_xor(acc.hash, d, kWave);
+            }
+            if (sub == 0) {
+                const uint32_t ho = (uint32_t)(off + min((int)(src[hl] & 0xFFFFu), n - 1));
+                if (w.wrap) finish_home<DIM, true, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+                else finish_home<DIM, false, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+                mine += (unsigned long long)acc.cnt;
+            }
+        }
+    } else {
+#pragma unroll 1
+        for (int h0 = first; h0 < hend; h0 += kSmallThreads) {
+            const int hl = h0 + tid;
+            if (hl < hend) {
+                Acc acc;
+                acc.x = acc.y = acc.z = 0.0;
+                acc.cnt = 0;
+                acc.hash = 0ull;
+                const uint32_t hx = SX[hl], hy = SY[hl], hz = DIM == 3 ? SZ[hl] : 0u;
+                const int htype = ST[hl];
+                const uint32_t ho = (uint32_t)(off + min((int)(src[hl] & 0xFFFFu), n - 1));
+                small_home<DIM, DEBUG, MLDS, 1>(a, w, M, SX, SY, SZ, ST, src, cst, hl, 0, hx, hy, hz,
+                                                htype * w.ntypes, acc);
+                if (w.wrap) finish_home<DIM, true, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+                else finish_home<DIM, false, DEBUG>(a, w, acc, hx, hy, hz, htype, ho, off + hl);
+                mine += (unsigned long long)acc.cnt;
+            }
         }
     }
     const unsigned long long tot = block_sum_u64<kSmallThreads>(mine, s_q);
