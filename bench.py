@@ -328,6 +328,7 @@ def main() -> int:
     eng.sync()
     wall = time.perf_counter() - t0
     pairs_last = eng.pair_count()
+    one_launch = eng.tuning()["small_step"] == 2
 
     # per-phase timing (events around each phase) on further steps
     prof = eng.profile(args.profile_steps)
@@ -403,13 +404,17 @@ def main() -> int:
             "workload": workloads[args.config],
             "n": w.n, "dim": w.dim, "ntypes": w.ntypes, "rmax_unit_box": w.rmax, "dt": w.dt,
         },
+        "step_path": ("one launch per step (k_step_small: keys, sort, cell table and force in one "
+                      "workgroup's LDS)" if one_launch else
+                      "multi-launch (bin, scans, radix sort, force)"),
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
         "app_path": {"ms_per_frame": frame_ms, "get_state_ms": get_state_ms,
                      "how": "update(1) + plife_get_particles of positions, velocities and types "
                             "into double[3N]/int[N] arrays reused across frames (as "
                             "PhysicsSnapshot.write), wall clock"},
         "roofline": {
-            "kernel": f"k_force<{w.dim}, wrap, no-debug>",
+            "kernel": (f"k_step_small<{w.dim}, no-debug> (the whole step)" if one_launch
+                       else f"k_force<{w.dim}, wrap, no-debug>"),
             "bound": "valu",            # vector ALU issue: sparse gather, no matrix ops
             "pipe": "FP32 screen + FP64 force law as vector instructions; priced against the "
                     "FP32 vector peak",

@@ -71,6 +71,7 @@ constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
+constexpr int kTuneSteps = 12;         // an update long enough to time both step paths
 constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
@@ -1198,12 +1199,12 @@ bool plan_small(plife_physics *h)
         int cus = 256;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess)
             cus = 256;
-        for (homes = 256;; homes *= 2) {
+        for (homes = 128;; homes *= 2) {
             long long blocks = 0;
             for (const WorldHost &w : h->worlds) blocks += std::max(1, ceil_div(w.n, homes));
             if (blocks <= cus || homes >= nmax) break;
         }
-        homes = std::max(256, std::min(homes, ((nmax + 255) / 256) * 256));
+        homes = std::max(128, std::min(homes, ((nmax + 127) / 128) * 128));
     }
     h->small_homes = homes;
     h->small_nmax = nmax;
@@ -1639,28 +1640,51 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         if ((rc = prepare_step(h))) return rc;
     }
     // Automatic tuning of a layout that fits the one-launch step: its first
-    // update of at least 6 steps times two steps on each path (after one
-    // warm-up step each; both paths give the same bits, so these are steps
-    // of the update) and keeps the faster path for the layout.  The redundant
-    // per-workgroup sort makes the one-launch step faster for small worlds
-    // and batches that leave CUs idle, slower once the device is full.
-    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= 6) {
+    // update of at least kTuneSteps steps times four steps on each path, run
+    // as the update would run them (replays of the step graph, or direct
+    // launches under direct_launch), after a warm-up step each (two where the
+    // one-launch step must come back to key buffer 0 for its graph); both
+    // paths give the same bits, so these are steps of the update.  The faster
+    // path is kept for the layout.  The redundant per-workgroup sort makes the
+    // one-launch step faster for small worlds and batches that leave CUs idle,
+    // slower once the device is full.
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= kTuneSteps) {
         float t[2] = {0.0f, 0.0f};
+        const bool graphs = !h->tune.direct_launch;
         for (int p = 0; p < 2; ++p) {
             if (honour_stop && h->stop.load()) {
                 if (k > 0) (void)copy_stats(h);
                 return PLIFE_STOPPED;
             }
             if ((rc = select_path(h, p == 1))) return rc;
+            int warm = 1;
             if ((rc = do_step(h, nullptr))) return rc;
-            HIP_TRY(hipEventRecord(h->tune_ev[0], h->stream));
-            for (int i = 0; i < 2; ++i)
+            if (graphs && h->small && h->small_kbuf != 0) {
                 if ((rc = do_step(h, nullptr))) return rc;
+                ++warm;
+            }
+            k += warm;
+            if ((rc = launched(warm))) return rc;
+            if (graphs) {
+                std::vector<unsigned char> key = graph_key(h);
+                if (!h->gexec || key != h->gkey) {
+                    if ((rc = capture_two_steps(h))) return rc;
+                    h->gkey = key;
+                    h->seen_key = key;
+                }
+            }
+            HIP_TRY(hipEventRecord(h->tune_ev[0], h->stream));
+            if (graphs) {
+                for (int i = 0; i < 2; ++i) HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+            } else {
+                for (int i = 0; i < 4; ++i)
+                    if ((rc = do_step(h, nullptr))) return rc;
+            }
             HIP_TRY(hipEventRecord(h->tune_ev[1], h->stream));
             HIP_TRY(hipEventSynchronize(h->tune_ev[1]));
             HIP_TRY(hipEventElapsedTime(&t[p], h->tune_ev[0], h->tune_ev[1]));
-            k += 3;
-            if ((rc = launched(3))) return rc;
+            k += 4;
+            if ((rc = launched(4))) return rc;
         }
         h->small_pref = t[1] < t[0];
         h->small_meas = h->version;
@@ -1719,8 +1743,8 @@ int time_steps(plife_physics *h, int nsteps, double *ms)
     // update of a new configuration launches directly, the second captures
     if ((rc = prepare_step(h))) return rc;
     // the automatic path choice of this layout, also outside the timed region
-    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= 6) {
-        if ((rc = run_steps(h, 6, false))) return rc;
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= kTuneSteps) {
+        if ((rc = run_steps(h, kTuneSteps, false))) return rc;
         if ((rc = prepare_step(h))) return rc;
     }
     for (int i = 0; i < 2 && nsteps >= 2 && !h->tune.direct_launch &&

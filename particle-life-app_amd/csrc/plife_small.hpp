@@ -42,6 +42,7 @@ constexpr int kSmallWaves = kSmallThreads / kWave;
 constexpr int kSmallDigitBits = 7;           // radix digit of the in-LDS sort
 constexpr int kSmallPer = 16;                // particles per thread at most
 constexpr int kSmallU = 4;                   // loop unroll: loads issued together
+constexpr int kSmallL = 8;                   // the same for the passes over the particles
 constexpr int kSmallMaxN = kSmallThreads * kSmallPer;   // also below 2^16: entries pack (key << 16 | index)
 constexpr int kSmallMaxCells = 1 << 14;      // keys of at most 14 bits: two radix passes
 
@@ -397,8 +398,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
     const bool lead = first == 0;                // writes the world's sort by-products
     int kbits = 0;
     while ((1 << kbits) < nc) ++kbits;
-    // groups of kSmallU entries i = tid + (r0 + u) kSmallThreads cover [0, n)
-    const int ngroups = (n + kSmallU * kSmallThreads - 1) / (kSmallU * kSmallThreads);
+    // rounds of kSmallL entries i = tid + (r0 + u) kSmallThreads cover [0, n)
+    const int nrounds = (n + kSmallL * kSmallThreads - 1) / (kSmallL * kSmallThreads);
 
     if constexpr (MLDS)
         for (int t = tid; t < w.ntypes * w.ntypes; t += kSmallThreads) M[t] = a.matrix[w.moff + t] * w.R;
@@ -412,19 +413,19 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
         bool bad = false;
         const double ics = 1.0 / w.cs;
 #pragma unroll 1
-        for (int g = 0; g < ngroups; ++g) {
-            uint32_t px[kSmallU], py[kSmallU], pz[kSmallU], po[kSmallU];
+        for (int g = 0; g < nrounds; ++g) {
+            uint32_t px[kSmallL], py[kSmallL], pz[kSmallL], po[kSmallL];
 #pragma unroll
-            for (int u = 0; u < kSmallU; ++u) {
-                const int i = off + min(tid + (g * kSmallU + u) * kSmallThreads, n - 1);
+            for (int u = 0; u < kSmallL; ++u) {
+                const int i = off + min(tid + (g * kSmallL + u) * kSmallThreads, n - 1);
                 px[u] = a.sx[i];
                 py[u] = a.sy[i];
                 pz[u] = DIM == 3 ? a.sz[i] : 0u;
                 po[u] = old_ok ? s.skeys_in[i] : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < kSmallU; ++u) {
-                const int i = tid + (g * kSmallU + u) * kSmallThreads;
+            for (int u = 0; u < kSmallL; ++u) {
+                const int i = tid + (g * kSmallL + u) * kSmallThreads;
                 if (i < n) {
                     uint32_t key = (uint32_t)key_axis_fast(px[u], w.cs, ics, nx) +
                                    (uint32_t)key_axis_fast(py[u], w.cs, ics, nx) * (uint32_t)nx;
@@ -452,10 +453,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
     if (old_ok) {
         bool bad = false;
 #pragma unroll 1
-        for (int g = 0; g < ngroups; ++g) {
+        for (int g = 0; g < nrounds; ++g) {
 #pragma unroll
-            for (int u = 0; u < kSmallU; ++u) {
-                const int i = tid + (g * kSmallU + u) * kSmallThreads;
+            for (int u = 0; u < kSmallL; ++u) {
+                const int i = tid + (g * kSmallL + u) * kSmallThreads;
                 const int ic = max(min(i, n - 1), 1);
                 bad |= (i >= 1) & (i < n) & (O[ic] < O[ic - 1]);
             }
@@ -503,10 +504,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
             if (m <= kSmallThreads) small_rank_sort(MV, MV2, m);
             else mv = small_radix_sort(MV, MV2, m, kbits, cnt, s_w);
             __syncthreads();
-            // A mover of sorted rank q, key c and slot i lands after every
-            // stayer of a smaller key or of key c and a smaller slot -- the
-            // stayers before slot i clamped to O's run of c -- and after the q
-            // movers before it.  Positions first: O shares P1 with the output.
+            // A mover of sorted rank q, key c and slot i lands after S_q
+            // stayers -- those before slot i clamped to O's run of c: every
+            // stayer of a smaller key, or of key c and a smaller slot -- and
+            // after the q movers before it.  Positions first: O shares P1 with
+            // the output.  The two searches of O's run go side by side.
+            uint32_t *Sq = mv == MV2 ? MV : MV2;     // S_q, non-decreasing in q
             int mpos[kSmallPer];
 #pragma unroll
             for (int r = 0; r < kSmallPer; ++r) {
@@ -515,42 +518,51 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
                 if (q < m) {
                     const uint32_t ee = mv[q];
                     const int c = (int)(ee >> 16), i = (int)(ee & 0xFFFFu);
-                    int lo = 0, hi = n;              // first slot with O >= c
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if ((int)O[mid] < c) lo = mid + 1;
-                        else hi = mid;
+                    int lo = 0, hi = n, lo2 = 0, hi2 = n;    // first slot with O >= c; with O > c
+                    while (lo < hi || lo2 < hi2) {
+                        const int mid = (lo + hi) >> 1, mid2 = (lo2 + hi2) >> 1;
+                        const int o1 = (int)O[min(mid, n - 1)], o2 = (int)O[min(mid2, n - 1)];
+                        if (lo < hi) {
+                            if (o1 < c) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        if (lo2 < hi2) {
+                            if (o2 <= c) lo2 = mid2 + 1;
+                            else hi2 = mid2;
+                        }
                     }
-                    int lo2 = lo, hi2 = n;           // first slot with O > c
-                    while (lo2 < hi2) {
-                        const int mid = (lo2 + hi2) >> 1;
-                        if ((int)O[mid] <= c) lo2 = mid + 1;
-                        else hi2 = mid;
-                    }
-                    mpos[r] = (int)Sp[min(max(i, lo), lo2)] + q;
+                    const int sq = (int)Sp[min(max(i, lo), lo2)];
+                    Sq[q] = (uint32_t)sq;
+                    mpos[r] = sq + q;
                 }
             }
             __syncthreads();
-            // A stayer of key c at slot i lands after the stayers before it
-            // (all of a smaller key, or of key c and a smaller slot) and the
-            // movers of a smaller key, or of key c and a smaller slot.
+            // A stayer of stayer rank k (= Sp[i] at its slot i) lands after the
+            // k stayers before it and the movers with S_q <= k.  Each thread
+            // places the stayers of its chunk [i0, i1) in slot order, counting
+            // those movers as k grows from a search at the chunk's first rank.
+            {
+                const int k0 = (int)Sp[i0];
+                int cm = 0, chi = m;
+                while (cm < chi) {
+                    const int mid = (cm + chi) >> 1;
+                    if ((int)Sq[mid] <= k0) cm = mid + 1;
+                    else chi = mid;
+                }
 #pragma unroll 1
-            for (int g = 0; g < ngroups; ++g) {
+                for (int r0 = 0; r0 < ch; r0 += kSmallU) {
+                    uint32_t e[kSmallU];
+                    int sp[kSmallU + 1];
 #pragma unroll
-                for (int u = 0; u < kSmallU; ++u) {
-                    const int i = tid + (g * kSmallU + u) * kSmallThreads;
-                    const int ic = min(i, n - 1);
-                    const uint32_t ee = P0[ic];
-                    if (i < n && Sp[ic + 1] != Sp[ic]) {
-                        const int c = (int)(ee >> 16);
-                        int lo = 0, hi = m;          // movers of a smaller key
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if ((int)(mv[mid] >> 16) < c) lo = mid + 1;
-                            else hi = mid;
+                    for (int u = 0; u <= kSmallU; ++u) sp[u] = (int)Sp[min(i0 + r0 + u, n)];
+#pragma unroll
+                    for (int u = 0; u < kSmallU; ++u) e[u] = P0[min(i0 + r0 + u, n - 1)];
+#pragma unroll
+                    for (int u = 0; u < kSmallU; ++u) {
+                        if (i0 + r0 + u < i1 && sp[u + 1] != sp[u]) {
+                            while (cm < m && (int)Sq[cm] <= sp[u]) ++cm;
+                            P1[sp[u] + cm] = e[u];
                         }
-                        while (lo < m && mv[lo] < ((uint32_t)c << 16 | (uint32_t)i)) ++lo;
-                        P1[(int)Sp[ic] + lo] = ee;
                     }
                 }
             }
@@ -579,13 +591,13 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
     // 4. sorted positions and types (the gather of Physics.java:122-131's
     //    particle copy) and the sort's by-products
 #pragma unroll 1
-    for (int g = 0; g < ngroups; ++g) {
-        uint32_t gx[kSmallU], gy[kSmallU], gz[kSmallU], e[kSmallU];
-        int gt[kSmallU];
+    for (int g = 0; g < nrounds; ++g) {
+        uint32_t gx[kSmallL], gy[kSmallL], gz[kSmallL], e[kSmallL];
+        int gt[kSmallL];
 #pragma unroll
-        for (int u = 0; u < kSmallU; ++u) e[u] = src[min(tid + (g * kSmallU + u) * kSmallThreads, n - 1)];
+        for (int u = 0; u < kSmallL; ++u) e[u] = src[min(tid + (g * kSmallL + u) * kSmallThreads, n - 1)];
 #pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
+        for (int u = 0; u < kSmallL; ++u) {
             const int go = off + min((int)(e[u] & 0xFFFFu), n - 1);
             gx[u] = a.sx[go];
             gy[u] = a.sy[go];
@@ -593,8 +605,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_step_small(SmallArgs s)
             gt[u] = a.type[go];
         }
 #pragma unroll
-        for (int u = 0; u < kSmallU; ++u) {
-            const int j = tid + (g * kSmallU + u) * kSmallThreads;
+        for (int u = 0; u < kSmallL; ++u) {
+            const int j = tid + (g * kSmallL + u) * kSmallThreads;
             if (j < n) {
                 SX[j] = gx[u];
                 SY[j] = gy[u];

@@ -91,15 +91,15 @@ def test_homes_per_workgroup(homes):
 
 
 def test_automatic_choice():
-    """Automatic tuning times both paths on the first update of at least 6
+    """Automatic tuning times both paths on the first update of at least 12
     steps of a layout that fits the one-launch step and keeps the faster
     (either way the bits of the multi-launch path); a world that does not fit,
     or another launch-shape field set, takes the multi-launch path."""
     print("\n[small] case=automatic_choice", flush=True)
     w = _world(117, 5_000, 6, 0.03, 0.02)
-    auto = _run(w, 9)
+    auto = _run(w, 14)
     assert auto["mode"] in (1, 2)
-    _assert_same(auto, _run(w, 9, small_step=1), "automatic")
+    _assert_same(auto, _run(w, 14, small_step=1), "automatic")
     assert _run(_world(118, 30_000, 6, 0.02, 0.02), 8)["mode"] == 1
     assert _run(w, 8, list_cap=16)["mode"] == 1
 
