@@ -214,7 +214,12 @@ int plife_get_accelerations(plife_physics *h, int32_t n, float *acc);
  * workgroups steps).  Results do not depend on the tuning.  Automatic
  * fields follow the world's clustering: every step measures the mean size of
  * the cell a particle sits in, and each plife_update plans its launches from
- * the latest measurement that has reached the host. */
+ * the latest measurement that has reached the host.  Two automatic choices
+ * are measured instead, on the first update of at least 24 steps: the step
+ * path (small_step 0, where every world fits) and, on the multi-launch path,
+ * block_homes (256 or 128 homes per workgroup; kept per class of layout:
+ * dimension, clustering, mean particles per cell, particle count).  Both
+ * time the candidates on steps of that update, which the update counts. */
 typedef struct {
     int32_t block_homes;
     int32_t list_cap;

@@ -71,7 +71,7 @@ constexpr long long kMaxCells = 1LL << 28;
 constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
-constexpr int kTuneSteps = 12;         // an update long enough to time both step paths
+constexpr int kTuneSteps = 24;         // an update long enough to time two configurations
 constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
@@ -303,6 +303,12 @@ struct plife_physics {
     int small_kbuf = 0;                // which of skeys (0) and kA (1) holds the last step's sorted keys
     unsigned long long *d_facc = nullptr;   // its counter accumulator (+ ticket), zero between steps
     size_t facc_cap = 0;
+
+    // homes per force workgroup of the multi-launch path where block_homes is
+    // automatic: measured once per class of layout (shape_class) and kept
+    int auto_bs = 0;                   // the choice for layout version bs_meas (0: the planner's own)
+    uint64_t bs_meas = ~0ull;
+    std::vector<std::pair<uint64_t, int>> bs_seen;   // (class, homes) of the classes measured
 
     PosBuf P(int k) { return PosBuf{px[k], py[k], pz[k]}; }
     AuxBuf A(int k) { return AuxBuf{vx[k], vy[k], vz[k], type[k], id[k]}; }
@@ -984,6 +990,15 @@ ForcePlan plan_force_for(int dim, double avg, double occ, bool clustered, const 
     return p;
 }
 
+// The tuning the planner follows: the caller's, with the measured homes per
+// workgroup where block_homes is automatic.
+plife_tuning plan_tuning(const plife_physics *h)
+{
+    plife_tuning t = h->tune;
+    if (t.block_homes == 0 && h->auto_bs != 0 && h->bs_meas == h->version) t.block_homes = h->auto_bs;
+    return t;
+}
+
 ForcePlan plan_force(const plife_physics *h)
 {
     const int dim = h->dim == 3 ? 3 : 2;
@@ -1000,7 +1015,38 @@ ForcePlan plan_force(const plife_physics *h)
     }
     const double avg = (double)sn / (double)std::max(sc, 1LL);
     const double occ = h->occupancy > 0.0 ? h->occupancy : avg + 1.0;   // Poisson: avg + 1
-    return plan_force_for(dim, avg, occ, h->clustered, h->tune, h->debug != 0, h->ntypes_max);
+    return plan_force_for(dim, avg, occ, h->clustered, plan_tuning(h), h->debug != 0, h->ntypes_max);
+}
+
+// The class of a layout for the measured homes per workgroup: dimension,
+// clustering, batch or single, the staged worlds' mean particles per cell in
+// half octaves and the particle count in octaves.
+uint64_t shape_class(const plife_physics *h)
+{
+    long long sn = 0, sc = 0;
+    for (const WorldHost &w : h->worlds)
+        if (!w.sparse) {
+            sn += w.n;
+            sc += grid_of(w.st).ncells;
+        }
+    const double avg = sc > 0 ? (double)sn / (double)sc : 0.0;
+    const int da = (int)std::floor(std::log2(std::max(avg, 1.0 / 64.0)) * 2.0) + 64;
+    const int nn = (int)std::floor(std::log2((double)std::max(h->n, 1)));
+    return (uint64_t)(h->dim == 3) | (uint64_t)h->clustered << 1 | (uint64_t)(h->batch != 0) << 2 |
+           (uint64_t)(uint32_t)da << 8 | (uint64_t)(uint32_t)nn << 24;
+}
+
+// Whether the next long update measures the homes per workgroup: the
+// multi-launch path with block_homes automatic, some world on the staged
+// force kernel, and no measurement for this layout or its class yet.
+bool shape_tune_due(const plife_physics *h)
+{
+    if (h->small || h->tune.block_homes != 0 || h->tune.small_step == 2 || h->n <= 0 ||
+        h->bs_meas == h->version)
+        return false;
+    for (const WorldHost &w : h->worlds)
+        if (!w.sparse && w.n > 0) return true;
+    return false;
 }
 
 // Read the clustering measure of the latest step the host has seen complete
@@ -1112,7 +1158,8 @@ void classify_worlds(plife_physics *h)
     const int dim = h->dim == 3 ? 3 : 2;
     const int NR = dim == 3 ? 9 : 3;
     const int mode = h->tune.sparse_worlds;
-    const double bs = h->tune.block_homes ? h->tune.block_homes : (h->clustered ? 128 : 256);
+    const plife_tuning pt = plan_tuning(h);
+    const double bs = pt.block_homes ? pt.block_homes : (h->clustered ? 128 : 256);
     bool changed = false;
     for (WorldHost &w : h->worlds) {
         const Grid gr = grid_of(w.st);
@@ -1262,6 +1309,16 @@ int prepare_step(plife_physics *h)
     if ((rc = sync_layout(h))) return rc;
     refresh_occupancy(h);
     classify_worlds(h);
+    // a layout of a class measured before takes that class's homes per workgroup
+    if (h->tune.block_homes == 0 && h->bs_meas != h->version) {
+        const uint64_t c = shape_class(h);
+        for (const auto &e : h->bs_seen)
+            if (e.first == c) {
+                h->auto_bs = e.second;
+                h->bs_meas = h->version;
+                classify_worlds(h);
+            }
+    }
     h->plan = plan_force(h);
     if ((rc = ensure_cells(h, h->ncells))) return rc;
     if ((rc = ensure_debug(h))) return rc;
@@ -1639,56 +1696,89 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         HIP_TRY(hipStreamSynchronize(h->stream));
         if ((rc = prepare_step(h))) return rc;
     }
-    // Automatic tuning of a layout that fits the one-launch step: its first
-    // update of at least kTuneSteps steps times four steps on each path, run
-    // as the update would run them (replays of the step graph, or direct
-    // launches under direct_launch), after a warm-up step each (two where the
-    // one-launch step must come back to key buffer 0 for its graph); both
-    // paths give the same bits, so these are steps of the update.  The faster
-    // path is kept for the layout.  The redundant per-workgroup sort makes the
-    // one-launch step faster for small worlds and batches that leave CUs idle,
-    // slower once the device is full.
-    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= kTuneSteps) {
-        float t[2] = {0.0f, 0.0f};
-        const bool graphs = !h->tune.direct_launch;
-        for (int p = 0; p < 2; ++p) {
-            if (honour_stop && h->stop.load()) {
-                if (k > 0) (void)copy_stats(h);
-                return PLIFE_STOPPED;
-            }
-            if ((rc = select_path(h, p == 1))) return rc;
-            int warm = 1;
+    // Four steps of the prepared configuration timed as the update runs them
+    // (two replays of the step graph, or direct launches under
+    // direct_launch), after a warm-up step (two where the one-launch step must
+    // come back to key buffer 0 for its graph).  Every configuration gives the
+    // same bits, so these are steps of the update.
+    const bool graphs = !h->tune.direct_launch;
+    auto time_four = [&](float *ms) -> int {
+        int warm = 1;
+        if ((rc = do_step(h, nullptr))) return rc;
+        if (graphs && h->small && h->small_kbuf != 0) {
             if ((rc = do_step(h, nullptr))) return rc;
-            if (graphs && h->small && h->small_kbuf != 0) {
-                if ((rc = do_step(h, nullptr))) return rc;
-                ++warm;
-            }
-            k += warm;
-            if ((rc = launched(warm))) return rc;
-            if (graphs) {
-                std::vector<unsigned char> key = graph_key(h);
-                if (!h->gexec || key != h->gkey) {
-                    if ((rc = capture_two_steps(h))) return rc;
-                    h->gkey = key;
-                    h->seen_key = key;
-                }
-            }
-            HIP_TRY(hipEventRecord(h->tune_ev[0], h->stream));
-            if (graphs) {
-                for (int i = 0; i < 2; ++i) HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
-            } else {
-                for (int i = 0; i < 4; ++i)
-                    if ((rc = do_step(h, nullptr))) return rc;
-            }
-            HIP_TRY(hipEventRecord(h->tune_ev[1], h->stream));
-            HIP_TRY(hipEventSynchronize(h->tune_ev[1]));
-            HIP_TRY(hipEventElapsedTime(&t[p], h->tune_ev[0], h->tune_ev[1]));
-            k += 4;
-            if ((rc = launched(4))) return rc;
+            ++warm;
         }
+        k += warm;
+        if ((rc = launched(warm))) return rc;
+        if (graphs) {
+            std::vector<unsigned char> key = graph_key(h);
+            if (!h->gexec || key != h->gkey) {
+                if ((rc = capture_two_steps(h))) return rc;
+                h->gkey = key;
+                h->seen_key = key;
+            }
+        }
+        HIP_TRY(hipEventRecord(h->tune_ev[0], h->stream));
+        if (graphs) {
+            for (int i = 0; i < 2; ++i) HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+        } else {
+            for (int i = 0; i < 4; ++i)
+                if ((rc = do_step(h, nullptr))) return rc;
+        }
+        HIP_TRY(hipEventRecord(h->tune_ev[1], h->stream));
+        HIP_TRY(hipEventSynchronize(h->tune_ev[1]));
+        HIP_TRY(hipEventElapsedTime(ms, h->tune_ev[0], h->tune_ev[1]));
+        k += 4;
+        return launched(4);
+    };
+    // Two configurations timed in the order A B B A (time_four each), so
+    // that a clock still rising at the start of the update, or any drift,
+    // weighs on both alike; t[c] sums configuration c's two timings.
+    auto time_pair = [&](auto &&configure, float t[2]) -> int {
+        static constexpr int order[4] = {0, 1, 1, 0};
+        t[0] = t[1] = 0.0f;
+        for (int i = 0; i < 4; ++i) {
+            if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
+            if (i == 0 || order[i] != order[i - 1])
+                if ((rc = configure(order[i]))) return rc;
+            float ms = 0.0f;
+            if ((rc = time_four(&ms))) return rc;
+            t[order[i]] += ms;
+        }
+        return PLIFE_OK;
+    };
+    // Automatic tuning of a layout that fits the one-launch step: its first
+    // update of at least kTuneSteps steps times both paths (time_pair) and
+    // keeps the faster for the layout.  The redundant per-workgroup sort makes
+    // the one-launch step faster for small worlds and batches that leave CUs
+    // idle, slower once the device is full.
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= kTuneSteps) {
+        float t[2];
+        rc = time_pair([&](int p) { return select_path(h, p == 1); }, t);
+        if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
+        if (rc) return rc;
         h->small_pref = t[1] < t[0];
         h->small_meas = h->version;
         if ((rc = select_path(h, h->small_pref))) return rc;
+    }
+    // Automatic homes per force workgroup on the multi-launch path: the first
+    // update of at least kTuneSteps steps of a class of layout (shape_class)
+    // times 256 and 128 homes per workgroup (time_pair) and keeps the faster
+    // for every layout of the class.
+    if (shape_tune_due(h) && nsteps - k >= kTuneSteps) {
+        static constexpr int cand[2] = {256, 128};
+        float t[2];
+        rc = time_pair([&](int p) {
+            h->auto_bs = cand[p];
+            h->bs_meas = h->version;
+            return prepare_step(h);
+        }, t);
+        if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
+        if (rc) return rc;
+        h->auto_bs = t[1] < t[0] ? cand[1] : cand[0];
+        h->bs_seen.emplace_back(shape_class(h), h->auto_bs);
+        if ((rc = prepare_step(h))) return rc;
     }
     bool use_graph = nsteps - k >= 2 && !h->tune.direct_launch;
     // the one-launch step alternates two key buffers; its graphs start from
@@ -1744,6 +1834,10 @@ int time_steps(plife_physics *h, int nsteps, double *ms)
     if ((rc = prepare_step(h))) return rc;
     // the automatic path choice of this layout, also outside the timed region
     if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= kTuneSteps) {
+        if ((rc = run_steps(h, kTuneSteps, false))) return rc;
+        if ((rc = prepare_step(h))) return rc;
+    }
+    if (shape_tune_due(h) && nsteps >= kTuneSteps) {
         if ((rc = run_steps(h, kTuneSteps, false))) return rc;
         if ((rc = prepare_step(h))) return rc;
     }
@@ -2825,7 +2919,10 @@ int plife_set_tuning(plife_physics *h, const plife_tuning *t)
     int rc = check_tuning(t);
     if (rc) return rc;
     h->tune = *t;
-    h->small_meas = ~0ull;             // re-time the paths under the new tuning
+    h->small_meas = ~0ull;             // re-time the paths and shapes under the new tuning
+    h->bs_meas = ~0ull;
+    h->auto_bs = 0;
+    h->bs_seen.clear();
     return PLIFE_OK;
 }
 

@@ -51,7 +51,8 @@ def _run(w, steps, **tune):
         c = e.cells()
         out.update(keys=c["keys"], cell_end=c["cell_end"], perm=c["perm"])
         out["nbr_count"], out["nbr_hash"] = e.neighbor_summary()
-        out["mode"] = e.tuning()["small_step"]
+        tn = e.tuning()
+        out["mode"], out["homes"] = tn["small_step"], tn["block_homes"]
     finally:
         e.close()
     return out
@@ -91,17 +92,44 @@ def test_homes_per_workgroup(homes):
 
 
 def test_automatic_choice():
-    """Automatic tuning times both paths on the first update of at least 12
+    """Automatic tuning times both paths on the first update of at least 24
     steps of a layout that fits the one-launch step and keeps the faster
     (either way the bits of the multi-launch path); a world that does not fit,
     or another launch-shape field set, takes the multi-launch path."""
     print("\n[small] case=automatic_choice", flush=True)
     w = _world(117, 5_000, 6, 0.03, 0.02)
-    auto = _run(w, 14)
+    auto = _run(w, 26)
     assert auto["mode"] in (1, 2)
-    _assert_same(auto, _run(w, 14, small_step=1), "automatic")
+    _assert_same(auto, _run(w, 26, small_step=1), "automatic")
     assert _run(_world(118, 30_000, 6, 0.02, 0.02), 8)["mode"] == 1
     assert _run(w, 8, list_cap=16)["mode"] == 1
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_automatic_homes_per_workgroup(dim):
+    """Where block_homes is automatic, the first update of at least 24 steps
+    of a layout on the multi-launch path times 256 and 128 homes per force
+    workgroup and keeps the faster; either way the bits are those of a fixed
+    launch shape, and a second world of the same class reuses the choice."""
+    from particle_life import Engine
+
+    print(f"\n[small] case=automatic_homes dim={dim}", flush=True)
+    n, rmax = (40_000, 0.02) if dim == 2 else (30_000, 0.06)     # too large for the one-launch step
+    w = _world(119, n, 6, rmax, 0.01, dim=dim)
+    auto = _run(w, 26)
+    assert auto["mode"] == 1 and auto["homes"] in (128, 256), (auto["mode"], auto["homes"])
+    _assert_same(auto, _run(w, 26, block_homes=256), "automatic homes")
+    _assert_same(auto, _run(w, 26, block_homes=128), "automatic homes vs 128")
+    e = Engine(0)
+    try:
+        e.load_world(w)
+        e.step(26)
+        first = e.tuning()["block_homes"]
+        e.load_world(_world(120, n, 6, rmax, 0.01, dim=dim))
+        e.step(3)                                    # too short to measure: the class's choice
+        assert e.tuning()["block_homes"] == first
+    finally:
+        e.close()
 
 
 @pytest.mark.parametrize("dim", [2, 3])
