@@ -39,7 +39,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
-    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=24)
     p.add_argument("--config", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
                    help="c2 (default): the 1M-particle 2D world the metric is quoted on; "
                         "c1: the 10k 2D world; c3: the clustered 4M 3D world; "
@@ -182,6 +182,7 @@ def bench_batch(args) -> int:
     ms = b.time_steps(args.steps)
     wall = time.perf_counter() - t0
     pairs = sum(b.info(i)["pairs"] for i in ids)
+    launch = b.tuning()
     prof = b.profile(args.profile_steps)
     b.close()
     steps_per_s = args.steps / (ms / 1e3)
@@ -209,6 +210,7 @@ def bench_batch(args) -> int:
                         "beta U[0.2, 0.4], half-life 0.04 s, dt 0.01 s, periodic",
             "n": ntot, "worlds": args.worlds,
         },
+        "launch": {k: launch[k] for k in ("block_homes", "list_cap", "small_step", "lds_bytes")},
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
     }
     if not args.no_plain:
@@ -328,7 +330,8 @@ def main() -> int:
     eng.sync()
     wall = time.perf_counter() - t0
     pairs_last = eng.pair_count()
-    one_launch = eng.tuning()["small_step"] == 2
+    launch = eng.tuning()
+    one_launch = launch["small_step"] == 2
 
     # per-phase timing (events around each phase) on further steps
     prof = eng.profile(args.profile_steps)
@@ -407,6 +410,7 @@ def main() -> int:
         "step_path": ("one launch per step (k_step_small: keys, sort, cell table and force in one "
                       "workgroup's LDS)" if one_launch else
                       "multi-launch (bin, scans, radix sort, force)"),
+        "launch": {k: launch[k] for k in ("block_homes", "list_cap", "small_step", "lds_bytes")},
         "phases_ms": {k: prof[k] for k in ("bin_ms", "sort_ms", "gather_ms", "force_ms", "step_ms")},
         "app_path": {"ms_per_frame": frame_ms, "get_state_ms": get_state_ms,
                      "how": "update(1) + plife_get_particles of positions, velocities and types "

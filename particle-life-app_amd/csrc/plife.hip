@@ -72,6 +72,7 @@ constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
 constexpr int kTuneSteps = 24;         // an update long enough to time two configurations
+constexpr int kTuneMax = 22;           // steps such a timing takes at most (4 x (warm-up + 4) + 2)
 constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
@@ -1753,7 +1754,8 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
     // keeps the faster for the layout.  The redundant per-workgroup sort makes
     // the one-launch step faster for small worlds and batches that leave CUs
     // idle, slower once the device is full.
-    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps - k >= kTuneSteps) {
+    if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= kTuneSteps &&
+        nsteps - k >= kTuneMax) {
         float t[2];
         rc = time_pair([&](int p) { return select_path(h, p == 1); }, t);
         if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
@@ -1766,7 +1768,7 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
     // update of at least kTuneSteps steps of a class of layout (shape_class)
     // times 256 and 128 homes per workgroup (time_pair) and keeps the faster
     // for every layout of the class.
-    if (shape_tune_due(h) && nsteps - k >= kTuneSteps) {
+    if (shape_tune_due(h) && nsteps >= kTuneSteps && nsteps - k >= kTuneMax) {
         static constexpr int cand[2] = {256, 128};
         float t[2];
         rc = time_pair([&](int p) {
