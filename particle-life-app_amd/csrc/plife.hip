@@ -1776,8 +1776,13 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
             h->bs_meas = h->version;
             return prepare_step(h);
         }, t);
-        if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
-        if (rc) return rc;
+        if (rc) {
+            // an interrupted measurement chooses nothing: the next long update measures again
+            h->auto_bs = 0;
+            h->bs_meas = ~0ull;
+            if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
+            return rc;
+        }
         h->auto_bs = t[1] < t[0] ? cand[1] : cand[0];
         h->bs_seen.emplace_back(shape_class(h), h->auto_bs);
         if ((rc = prepare_step(h))) return rc;
