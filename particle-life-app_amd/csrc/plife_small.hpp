@@ -19,13 +19,15 @@
 // visits its candidates in the order of the other force kernels (home_global:
 // per stencil row, cells left to right, each cell in sorted order) with the
 // same FP32 superset screen, FP64 force law and integrator, so the new state
-// is bit-identical to the multi-launch path's.
+// is bit-identical to the multi-launch path's (the lanes that share a home
+// add its terms in the same order; the FP32 screen only skips pairs the FP64
+// predicate rejects).
 //
 // A step is a chain of short phases on one CU, so per-thread loops are
-// unrolled four deep with clamped indices: each group of four loads is
-// issued together instead of one round trip per iteration.  Loops are not
-// unrolled further, and the boundary mode is a run-time flag, to keep the
-// code (run once per workgroup) small.
+// unrolled with clamped indices: the passes over the particles issue eight
+// loads per thread together, the other loops four, instead of one round trip
+// per iteration.  Loops are not unrolled further, and the boundary mode is a
+// run-time flag, to keep the code (run once per workgroup) small.
 //
 // Counters: workgroups add their pairs into an accumulator; the last
 // workgroup to finish (a ticket counter, no waiting) publishes the step's
