@@ -72,7 +72,7 @@ constexpr int kMaxWorlds = 65535;     // the world index of a slot is a u16
 constexpr int kMaxLds = 65536;
 constexpr int kClusteredLds = 17152;  // force-kernel LDS in clustered worlds: 9 workgroups per CU
 constexpr int kTuneSteps = 24;         // an update long enough to time two configurations
-constexpr int kTuneMax = 22;           // steps such a timing takes at most (4 x (warm-up + 4) + 2)
+constexpr int kTuneMax = 22;           // steps such a timing takes at most (4 x 4 + 3 switches + 3)
 constexpr size_t kWordLds = 24576;    // most LDS for the balanced path's mask words
 
 struct ForcePlan {
@@ -1699,19 +1699,16 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
     }
     // Four steps of the prepared configuration timed as the update runs them
     // (two replays of the step graph, or direct launches under
-    // direct_launch), after a warm-up step (two where the one-launch step must
-    // come back to key buffer 0 for its graph).  Every configuration gives the
-    // same bits, so these are steps of the update.
+    // direct_launch); the one-launch step first comes back to key buffer 0
+    // for its graph.  Every configuration gives the same bits, so these are
+    // steps of the update.
     const bool graphs = !h->tune.direct_launch;
     auto time_four = [&](float *ms) -> int {
-        int warm = 1;
-        if ((rc = do_step(h, nullptr))) return rc;
         if (graphs && h->small && h->small_kbuf != 0) {
             if ((rc = do_step(h, nullptr))) return rc;
-            ++warm;
+            k += 1;
+            if ((rc = launched(1))) return rc;
         }
-        k += warm;
-        if ((rc = launched(warm))) return rc;
         if (graphs) {
             std::vector<unsigned char> key = graph_key(h);
             if (!h->gexec || key != h->gkey) {
@@ -1733,25 +1730,37 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         k += 4;
         return launched(4);
     };
-    // Two configurations timed in the order A B B A (time_four each), so
-    // that a clock still rising at the start of the update, or any drift,
-    // weighs on both alike; t[c] sums configuration c's two timings.
+    // Two configurations, 0 (the default) and 1, timed in the order A B B A
+    // (time_four each), so that a clock still rising at the start of the
+    // update, or any drift, weighs on both alike; t[c] sums configuration c's
+    // two timings.  Each switch steps the configuration once and prepares it
+    // again, so that its workgroups are ordered by that step's measured work
+    // as they are in the update's later steps.
     auto time_pair = [&](auto &&configure, float t[2]) -> int {
         static constexpr int order[4] = {0, 1, 1, 0};
         t[0] = t[1] = 0.0f;
         for (int i = 0; i < 4; ++i) {
             if (honour_stop && h->stop.load()) return PLIFE_STOPPED;
-            if (i == 0 || order[i] != order[i - 1])
+            if (i == 0 || order[i] != order[i - 1]) {
                 if ((rc = configure(order[i]))) return rc;
+                if ((rc = do_step(h, nullptr))) return rc;
+                k += 1;
+                if ((rc = launched(1))) return rc;
+                if ((rc = copy_stats(h))) return rc;
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                if ((rc = configure(order[i]))) return rc;
+            }
             float ms = 0.0f;
             if ((rc = time_four(&ms))) return rc;
             t[order[i]] += ms;
         }
         return PLIFE_OK;
     };
+    // the alternative replaces the default only when measurably faster
+    auto faster = [](const float t[2]) { return t[1] < 0.97f * t[0]; };
     // Automatic tuning of a layout that fits the one-launch step: its first
-    // update of at least kTuneSteps steps times both paths (time_pair) and
-    // keeps the faster for the layout.  The redundant per-workgroup sort makes
+    // update of at least kTuneSteps steps times the multi-launch path against
+    // the one-launch step (time_pair) and keeps the faster for the layout.  The redundant per-workgroup sort makes
     // the one-launch step faster for small worlds and batches that leave CUs
     // idle, slower once the device is full.
     if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= kTuneSteps &&
@@ -1760,16 +1769,16 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
         rc = time_pair([&](int p) { return select_path(h, p == 1); }, t);
         if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
         if (rc) return rc;
-        h->small_pref = t[1] < t[0];
+        h->small_pref = faster(t);
         h->small_meas = h->version;
         if ((rc = select_path(h, h->small_pref))) return rc;
     }
     // Automatic homes per force workgroup on the multi-launch path: the first
     // update of at least kTuneSteps steps of a class of layout (shape_class)
-    // times 256 and 128 homes per workgroup (time_pair) and keeps the faster
-    // for every layout of the class.
+    // times the planner's choice (256, or 128 for clustered 3D) against the
+    // other (time_pair) and keeps the faster for every layout of the class.
     if (shape_tune_due(h) && nsteps >= kTuneSteps && nsteps - k >= kTuneMax) {
-        static constexpr int cand[2] = {256, 128};
+        const int cand[2] = {h->clustered ? 128 : 256, h->clustered ? 256 : 128};   // the planner's first
         float t[2];
         rc = time_pair([&](int p) {
             h->auto_bs = cand[p];
@@ -1783,7 +1792,7 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
             if (rc == PLIFE_STOPPED && k > 0) (void)copy_stats(h);
             return rc;
         }
-        h->auto_bs = t[1] < t[0] ? cand[1] : cand[0];
+        h->auto_bs = faster(t) ? cand[1] : cand[0];
         h->bs_seen.emplace_back(shape_class(h), h->auto_bs);
         if ((rc = prepare_step(h))) return rc;
     }

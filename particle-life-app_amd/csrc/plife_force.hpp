@@ -627,6 +627,11 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         bq = (int)fmin(2147483647.0, floor(rq * rq) + 1.0);
     }
 
+    // The force law's two additive constants held in VGPRs: each meets
+    // another scalar operand in an FMA, which reads one scalar at most.
+    WorldDev wf = w;
+    asm volatile("" : "+v"(wf.one_plus_beta));
+    asm volatile("" : "+v"(wf.inv_beta));
     // One candidate k of the staged window for a home at (px, py, pz) with
     // matrix row prow_m: the reference's FP64 predicate and force law.
     auto visit = [&](Acc &ac, int k, uint32_t px, uint32_t py, uint32_t pz, int prow_m) {
@@ -642,7 +647,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         }
         const int idx = prow_m + (int)st[k];
         const double m = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
-        pair64<DIM, DEBUG>(ac, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, w);
+        pair64<DIM, DEBUG>(ac, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, wf);
     };
     // Stages window [w0, w0 + wn) of the staged neighbourhood.
     auto stage = [&](int w0, int wn) {
