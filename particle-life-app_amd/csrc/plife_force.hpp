@@ -820,8 +820,7 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                 ab.hash = 0ull;
                 int left = s_hc[h], j = s_wo[h], kb = 0;
                 uint32_t m = 0;
-#pragma unroll 1
-                while (left > 0) {
+                auto next = [&]() {
                     if (m == 0) {
                         m = wmask[j];
                         kb = wbase[j];
@@ -829,9 +828,18 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
                     }
                     const int k = kb + __builtin_ctz(m);
                     m &= m - 1u;
-                    --left;
-                    visit(ab, k, hp.x, hp.y, bhz, bmrow);
+                    return k;
+                };
+                // two candidates per iteration (half the loop's bookkeeping;
+                // their force laws overlap), added in candidate order
+#pragma unroll 1
+                for (; left >= 2; left -= 2) {
+                    const int k1 = next();
+                    const int k2 = next();
+                    visit(ab, k1, hp.x, hp.y, bhz, bmrow);
+                    visit(ab, k2, hp.x, hp.y, bhz, bmrow);
                 }
+                if (left > 0) visit(ab, next(), hp.x, hp.y, bhz, bmrow);
                 const int gh = g0 + h;
                 finish_home<DIM, WRAP, DEBUG>(a, w, ab, hp.x, hp.y, bhz, btype, a.perm[gh], gh);
                 mycnt = (unsigned long long)ab.cnt;
