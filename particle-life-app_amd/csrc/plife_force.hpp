@@ -121,6 +121,28 @@ __device__ __forceinline__ void pair64(Acc &acc, int dxi, int dyi, int dzi, doub
     }
 }
 
+// pair64 with no branch: the coefficient of a pair out of range (or
+// rejected by the caller, ok = false) is selected to 0, which leaves the sums
+// bit-identical (a sum that starts at +0 is never -0), so that the chains of
+// neighbouring candidates can overlap.
+template <int DIM, bool DEBUG>
+__device__ __forceinline__ void pair64_sel(Acc &acc, bool ok, int dxi, int dyi, int dzi, double mR,
+                                           uint32_t qid, const WorldDev &a)
+{
+    const double dx = (double)dxi, dy = (double)dyi, dz = (double)dzi;
+    const double d2 = pair_d2<DIM>(dx, dy, dz);
+    const bool in = ok & (d2 != 0.0) & (d2 <= a.r2u);    // Physics.java:432
+    double cf = pair_coef(d2, mR, a);
+    asm volatile("" : "+v"(cf));        // computed for every candidate: no branch
+    const double c = in ? cf : 0.0;
+    acc.x = fma(c, dx, acc.x);
+    acc.y = fma(c, dy, acc.y);
+    if constexpr (DIM == 3) acc.z = fma(c, dz, acc.z);
+    acc.cnt += in ? 1 : 0;
+    if constexpr (DEBUG)
+        if (in) acc.hash += id_hash(qid);
+}
+
 // The block's pairs, for the host to order the next updates' blocks
 // heaviest first; entry nb_total tags the table they belong to.
 __device__ __forceinline__ void record_block_work(const ForceArgs &a, int blk,
@@ -638,16 +660,16 @@ __global__ __launch_bounds__(BS) void k_force(ForceArgs a)
         const uint2 q = sxy[k];
         const int dxi = (int)(q.x - px), dyi = (int)(q.y - py);
         const int dzi = DIM == 3 ? (int)(szz[k] - pz) : 0;
+        bool ok = true;
         if constexpr (!WRAP) {
             // plain difference of the clamped world: |D| >= 1/2 unless the
             // int32 difference has the sign of the unsigned order
-            bool ok = ((dxi >= 0) == (q.x >= px)) & ((dyi >= 0) == (q.y >= py));
+            ok = ((dxi >= 0) == (q.x >= px)) & ((dyi >= 0) == (q.y >= py));
             if (DIM == 3) ok &= ((dzi >= 0) == (szz[k] >= pz));
-            if (!ok) return;
         }
         const int idx = prow_m + (int)st[k];
         const double m = MLDS ? M[idx] : a.matrix[w.moff + idx] * w.R;
-        pair64<DIM, DEBUG>(ac, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, wf);
+        pair64_sel<DIM, DEBUG>(ac, ok, dxi, dyi, dzi, m, DEBUG ? sid[k] : 0u, wf);
     };
     // Stages window [w0, w0 + wn) of the staged neighbourhood.
     auto stage = [&](int w0, int wn) {
