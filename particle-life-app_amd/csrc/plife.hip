@@ -1760,9 +1760,10 @@ int run_steps(plife_physics *h, int nsteps, bool honour_stop)
     auto faster = [](const float t[2]) { return t[1] < 0.97f * t[0]; };
     // Automatic tuning of a layout that fits the one-launch step: its first
     // update of at least kTuneSteps steps times the multi-launch path against
-    // the one-launch step (time_pair) and keeps the faster for the layout.  The redundant per-workgroup sort makes
-    // the one-launch step faster for small worlds and batches that leave CUs
-    // idle, slower once the device is full.
+    // the one-launch step (time_pair) and keeps the faster for the layout.
+    // The redundant per-workgroup sort makes the one-launch step faster for
+    // small worlds and batches that leave CUs idle, slower once the device is
+    // full.
     if (h->small_ok && h->tune.small_step == 0 && h->small_meas != h->version && nsteps >= kTuneSteps &&
         nsteps - k >= kTuneMax) {
         float t[2];
